@@ -1,0 +1,244 @@
+"""MI355X-native batched per-patch Poisson-gravity solver.
+
+The product is the C ABI in include/pg_amd.h, implemented by libpgamd.so (hand-written HIP kernels for
+gfx950, built by ``make -C gpu_testbed_poisson_gravity_amd``).  This module is a thin ctypes binding of that
+ABI for Python callers (tests, bench.py); it has no compute path of its own and raises if the library is
+missing.  Device arrays are passed as raw pointers, e.g. ``tensor.data_ptr()`` of a torch tensor.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpgamd.so")
+
+SOR, MG = 1, 2
+INT_CQUAD, INT_QUAD = 4, 5
+INIT_INTERP, INIT_ZERO, INIT_GUESS = 0, 1, 2
+TERM_REFERENCE, TERM_RELRES = 0, 1
+STATUS_CONVERGED, STATUS_MAX_ITER, STATUS_NONFINITE = 1, 2, 4
+ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_NOMEM = -1, -2, -3, -4
+
+
+class PgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pgamd error {code}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("pot_ghost_size", C.c_int),
+        ("gra_ghost_size", C.c_int),
+        ("pot_scheme", C.c_int),
+        ("init_mode", C.c_int),
+        ("term_mode", C.c_int),
+        ("check_every", C.c_int),
+        ("rel_tol", C.c_float),
+        ("fuse_gravity", C.c_int),
+    ]
+
+
+class SolverArgs(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("dh", C.c_float),
+        ("sor_min_iter", C.c_int),
+        ("sor_max_iter", C.c_int),
+        ("sor_omega", C.c_float),
+        ("mg_max_iter", C.c_int),
+        ("mg_npre_smooth", C.c_int),
+        ("mg_npost_smooth", C.c_int),
+        ("mg_tolerated_error", C.c_float),
+        ("poi_coeff", C.c_float),
+        ("int_scheme", C.c_int),
+        ("p5_gradient", C.c_int),
+        ("poisson", C.c_int),
+        ("graacc", C.c_int),
+    ]
+
+
+# Every entry point of include/pg_amd.h with its ctypes signature.
+_FP = C.POINTER(C.c_float)
+_FPP = C.POINTER(_FP)
+_IP = C.POINTER(C.c_int)
+_CTX = C.c_void_p
+SIGNATURES = {
+    "pgamd_default_options": (None, [C.POINTER(Options)]),
+    "pgamd_set_diagnose_device": (C.c_int, [C.c_int, _IP, _IP, C.POINTER(C.c_void_p)]),
+    "pgamd_destroy": (C.c_int, [_CTX]),
+    "pgamd_set_options": (C.c_int, [_CTX, C.POINTER(Options)]),
+    "pgamd_last_error": (C.c_char_p, []),
+    "pgamd_device_info": (C.c_int, [_CTX, C.c_char_p, C.c_int]),
+    "pgamd_mem_allocate_poisson_gravity": (C.c_int, [_CTX, C.c_int, _FPP, _FPP, _FPP, _FPP]),
+    "pgamd_mem_free_poisson_gravity": (C.c_int, [_CTX, _FPP, _FPP, _FPP, _FPP]),
+    "pgamd_asyn_poisson_gravity_solver": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                                    C.c_float, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int,
+                                                    C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int,
+                                                    C.c_int, C.c_int, C.c_int]),
+    "pgamd_synchronize": (C.c_int, [_CTX]),
+    "pgamd_get_patch_stats": (C.c_int, [_CTX, _IP, _IP, C.c_int]),
+    "pgamd_solve_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.POINTER(SolverArgs), C.c_void_p, C.c_void_p, C.c_void_p]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libpgamd.so; raises FileNotFoundError when it has not been built (there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with `make -C {HERE}` (no CPU fallback exists)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def geometry(gs: int, gg: int = 1) -> dict:
+    n = 8 + 2 * gs
+    return {"N": n, "R": n - 2, "C": 4 + 2 * ((gs + 3) // 2), "G": 8 + 2 * gg}
+
+
+def _check(rc):
+    if rc != 0:
+        raise PgError(rc, load().pgamd_last_error().decode())
+
+
+def _vp(a):
+    """Raw pointer of a numpy array, torch tensor (data_ptr) or int; None stays NULL."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def solver_args(dt=1e-2, dh=1.0, sor_min_iter=10, sor_max_iter=60, sor_omega=1.69, mg_max_iter=10, mg_npre=3,
+                mg_npost=3, mg_tol=1e-6, poi_coeff=4.0 * np.pi * 1e-5, int_scheme=INT_CQUAD, p5=False, poisson=True,
+                graacc=True) -> SolverArgs:
+    return SolverArgs(dt, dh, sor_min_iter, sor_max_iter, sor_omega, mg_max_iter, mg_npre, mg_npost, mg_tol,
+                      poi_coeff, int_scheme, int(p5), int(poisson), int(graacc))
+
+
+class Context:
+    """One device context (pgamd_set_diagnose_device ... pgamd_destroy)."""
+
+    def __init__(self, device: int = 0, nstream: int = 1):
+        lib = load()
+        npg, ns = C.c_int(0), C.c_int(nstream)
+        h = C.c_void_p()
+        _check(lib.pgamd_set_diagnose_device(device, C.byref(npg), C.byref(ns), C.byref(h)))
+        self.h = h
+        self.default_npatch_group = npg.value
+        self.nstream = ns.value
+        self.opts = Options()
+        lib.pgamd_default_options(C.byref(self.opts))
+        self._host = None
+
+    def close(self):
+        if self.h:
+            if self._host is not None:
+                self.mem_free()
+            _check(load().pgamd_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_options(self, **kw):
+        for k, v in kw.items():
+            setattr(self.opts, k, v)
+        _check(load().pgamd_set_options(self.h, C.byref(self.opts)))
+
+    def device_info(self) -> str:
+        buf = C.create_string_buffer(256)
+        _check(load().pgamd_device_info(self.h, buf, 256))
+        return buf.value.decode()
+
+    # --- reference call shape ---------------------------------------------------------------------------
+    def mem_allocate(self, npatch_group: int):
+        """pgamd_mem_allocate_poisson_gravity; returns numpy views of the pinned host buffers."""
+        ptrs = [_FP() for _ in range(4)]
+        _check(load().pgamd_mem_allocate_poisson_gravity(self.h, npatch_group, *[C.byref(p) for p in ptrs]))
+        g = geometry(self.opts.pot_ghost_size, self.opts.gra_ghost_size)
+        n = 8 * npatch_group
+        shapes = [(n, g["R"], g["R"], g["R"]), (n, g["C"], g["C"], g["C"]), (n, g["G"], g["G"], g["G"]),
+                  (n, 5, 8, 8, 8)]
+        views = [np.ctypeslib.as_array(p, shape=s) for p, s in zip(ptrs, shapes)]
+        self._host = ptrs
+        return views
+
+    def mem_free(self):
+        ptrs = self._host or [_FP() for _ in range(4)]
+        _check(load().pgamd_mem_free_poisson_gravity(self.h, *[C.byref(p) for p in ptrs]))
+        self._host = None
+
+    def asyn_poisson_gravity_solver(self, h_rho, h_pot_in, h_pot_out, h_flu, npatch_group, dt, dh, sor_min_iter,
+                                    sor_max_iter, sor_omega, mg_max_iter, mg_npre_smooth, mg_npost_smooth,
+                                    mg_tolerated_error, poi_coeff, int_scheme, p5_gradient, poisson, graacc,
+                                    gpu_nstream=1):
+        for a in (h_rho, h_pot_in, h_pot_out, h_flu):
+            if a is not None and not (a.dtype == np.float32 and a.flags.c_contiguous):
+                raise TypeError("host arrays must be C-contiguous float32")
+        _check(load().pgamd_asyn_poisson_gravity_solver(
+            self.h, _vp(h_rho), _vp(h_pot_in), _vp(h_pot_out), _vp(h_flu), npatch_group, dt, dh, sor_min_iter,
+            sor_max_iter, sor_omega, mg_max_iter, mg_npre_smooth, mg_npost_smooth, mg_tolerated_error, poi_coeff,
+            int_scheme, int(p5_gradient), int(poisson), int(graacc), gpu_nstream))
+
+    def synchronize(self):
+        _check(load().pgamd_synchronize(self.h))
+
+    def patch_stats(self, npatch: int):
+        iters = np.zeros(npatch, np.int32)
+        status = np.zeros(npatch, np.int32)
+        _check(load().pgamd_get_patch_stats(self.h, iters.ctypes.data_as(_IP), status.ctypes.data_as(_IP), npatch))
+        return iters, status
+
+    # --- device-resident ----------------------------------------------------------------------------------
+    def solve_device(self, rho, cpot, pot_out, flu, npatch, args: SolverArgs, guess=None, iters=None, status=None,
+                     stream=None):
+        _check(load().pgamd_solve_device(self.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out), _vp(flu), npatch,
+                                         C.byref(args), _vp(iters), _vp(status), stream))
+
+
+def solve_host(rho, cpot, flu, *, gs, gg=1, scheme=SOR, init_mode=INIT_INTERP, term_mode=TERM_REFERENCE,
+               check_every=1, rel_tol=1e-5, fuse_gravity=True, guess=None, pot_out=None, device=0, **argkw):
+    """Convenience for tests: one batched solve through the reference-shaped host entry point (or the
+    device entry point when a guess is given, since the host shape has no guess argument).  Returns
+    (pot_out, flu_out, iters, status) as numpy arrays."""
+    import torch
+
+    args = solver_args(**argkw)
+    g = geometry(gs, gg)
+    npatch = (rho if rho is not None else pot_out).shape[0]
+    with Context(device) as ctx:
+        ctx.set_options(pot_ghost_size=gs, gra_ghost_size=gg, pot_scheme=scheme, init_mode=init_mode,
+                        term_mode=term_mode, check_every=check_every, rel_tol=rel_tol, fuse_gravity=int(fuse_gravity))
+        dev = torch.device("cuda", device)
+        t = lambda a: None if a is None else torch.as_tensor(np.ascontiguousarray(a, np.float32)).to(dev)  # noqa
+        d_rho, d_cpot, d_guess = t(rho), t(cpot), t(guess)
+        d_flu = t(flu) if flu is not None else None
+        d_out = (t(pot_out) if pot_out is not None else torch.zeros((npatch, g["G"], g["G"], g["G"]), device=dev))
+        d_it = torch.zeros(npatch, dtype=torch.int32, device=dev)
+        d_st = torch.zeros(npatch, dtype=torch.int32, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        ctx.solve_device(d_rho, d_cpot, d_out, d_flu, npatch, args, guess=d_guess, iters=d_it, status=d_st,
+                         stream=stream)
+        torch.cuda.synchronize(dev)
+        return (d_out.cpu().numpy(), None if d_flu is None else d_flu.cpu().numpy(), d_it.cpu().numpy(),
+                d_st.cpu().numpy())

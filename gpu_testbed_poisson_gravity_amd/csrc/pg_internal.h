@@ -1,0 +1,29 @@
+// Declarations shared between the C-ABI layer and the solver modules (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/pg_amd.h"
+#include "pg_kernels.h"
+
+namespace pgamd {
+
+int set_error(int code, const char* msg);
+int validate_options(const pgamd_options& o);
+int validate_args(const pgamd_options& o, const pgamd_solver_args& a, int npatch);
+SolveParams make_params(const pgamd_options& o, const pgamd_solver_args& a);
+int enqueue_solve(pgamd_context* c, const float* rho, const float* cpot, const float* guess, float* pot_out,
+                  float* flu, int npatch, const pgamd_solver_args& a, const SolveParams& p, int* iters, int* status,
+                  hipStream_t st);
+const pgamd_options& ctx_options(const pgamd_context* c);
+SorLayout& ctx_layout(pgamd_context* c);
+int ctx_cu_count(const pgamd_context* c);
+int ctx_device(const pgamd_context* c);
+hipStream_t ctx_stream(const pgamd_context* c);
+// Frees whatever the optional modules (graphs, mixed batches, CNN) attached to a context.
+void release_extras(pgamd_context* c);
+
+hipError_t launch_sor_uniform_dispatch(const UniformBatch& b, const SolveParams& p, int ppw, hipStream_t s);
+hipError_t launch_mg_uniform(const UniformBatch& b, const SolveParams& p, hipStream_t s);
+
+}  // namespace pgamd

@@ -1,0 +1,111 @@
+// Launch-side view of the batched Poisson-gravity kernels (host code includes this; no HIP types in the
+// structs so that the C-ABI layer and the autotuner can fill them directly).
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+namespace pgamd {
+
+// Uniform per-launch solver settings.  Floating-point constants are formed once on the host, in FP32, the
+// way the reference CPU solver forms them (CPU_PoissonSolver_SOR.cpp:51-52), so that device and CPU agree
+// bit for bit.
+struct SolveParams {
+  float cst;          // Poi_Coeff*dh*dh
+  float w6;           // omega/6
+  float damp;         // |1 - omega|, second-colour residual factor of the relative-residual rule
+  float rel_tol;      // relative-residual tolerance (term_mode 1)
+  int min_iter;
+  int max_iter;
+  int term_mode;      // 0: reference stall rule; 1: relative residual
+  int check_every;    // relative-residual cadence
+  int init_mode;      // 0: interpolated cube; 1: zero interior; 2: caller guess
+  int int_scheme;     // 4 CQUAD, 5 QUAD
+  int gra_ghost;      // GRA_GHOST_SIZE of the output window (1 or 2)
+  // multigrid
+  float poi_coeff;
+  float dh;
+  float mg_tol;
+  int mg_max_iter;
+  int mg_npre;
+  int mg_npost;
+  // gravity (fused after the solve when fuse_gravity != 0)
+  int fuse_gravity;
+  int p5;
+  float gra_const;
+};
+
+// Status bits written per patch.
+constexpr int kStatusConverged = 1;  // the stopping rule was met before max_iter
+constexpr int kStatusMaxIter = 2;    // max_iter reached
+constexpr int kStatusNonFinite = 4;  // a non-finite value is present in the returned potential
+
+// One batch of patches sharing a ghost size, with dense per-patch strides (the reference layout).
+struct UniformBatch {
+  const float* rho;    // [n][R^3]
+  const float* cpot;   // [n][C^3]
+  const float* guess;  // [n][R^3] or null
+  float* pot_out;      // [n][G^3]
+  float* flu;          // [n][5][8^3] or null
+  int* iters;          // [n] or null
+  int* status;         // [n] or null
+  int npatch;
+  int gs;
+  int scheme;
+};
+
+// A patch of a mixed batch: its ghost size, its method and element offsets into packed arrays.
+struct PatchDesc {
+  int gs;
+  int scheme;
+  int64_t rho_off;     // into the packed density (and guess) array
+  int64_t cpot_off;    // into the packed coarse-potential array
+};
+
+// Work item of the persistent kernel: up to `count` patches of one (gs, scheme) class, listed by index
+// in a permutation array.  Built on the host by the scheduler.
+struct Task {
+  int gs;
+  int scheme;
+  int first;           // index into the permutation array
+  int count;
+};
+
+struct MixedBatch {
+  const PatchDesc* desc;   // [npatch]
+  const int* perm;         // patch indices grouped by task
+  const Task* tasks;       // [ntasks], largest work first
+  int ntasks;
+  int npatch;
+  const float* rho;        // packed
+  const float* cpot;       // packed
+  const float* guess;      // packed like rho, or null
+  float* pot_out;          // [npatch][G^3]
+  float* flu;              // [npatch][5][8^3] or null
+  int* iters;
+  int* status;
+  unsigned int* counter;   // work-queue head, reset by the launcher
+};
+
+// Per-ghost-size launch layout of the SOR kernel: patches per wavefront.  Chosen by the autotuner;
+// the default is the densest packing that fits one wavefront.
+struct SorLayout {
+  int ppw[6];              // index by gs
+  int waves_per_cu;        // persistent kernel occupancy target
+};
+
+// Launchers (pg_kernels.hip).  All are asynchronous on `stream`.
+hipError_t launch_uniform(const UniformBatch& b, const SolveParams& p, const SorLayout& lay, hipStream_t stream);
+hipError_t launch_gravity(const float* pot, float* flu, int npatch, int gra_ghost, float gra_const, int p5,
+                          hipStream_t stream);
+hipError_t launch_mixed(const MixedBatch& b, const SolveParams* per_gs, const SorLayout& lay, int grid,
+                        hipStream_t stream);
+hipError_t launch_reset_counter(unsigned int* counter, hipStream_t stream);
+int sor_lanes_per_patch(int gs);
+int default_ppw(int gs);
+size_t lds_bytes_sor(int gs, int ppw);
+size_t lds_bytes_mg(int gs);
+int supported_ppw(int gs, int ppw);
+
+}  // namespace pgamd

@@ -1,0 +1,436 @@
+// Per-patch multigrid V-cycle on gfx950: one workgroup of 256 threads per patch, every level resident
+// in LDS, the level-0 right-hand side held in registers.
+//
+// Reference behaviour reproduced (bit for bit in FP32 for the same number of V-cycles):
+//   level schedule        CPU_PoissonSolver_MG.cpp:62-87      (GPU: CUPOT_PoissonSolver_MG.cu:16-53)
+//   V-cycle driver        CPU_PoissonSolver_MG.cpp:213-258    (GPU: CUPOT_PoissonSolver_MG.cu:414-463)
+//   smoother              CPU_PoissonSolver_MG.cpp:392-441
+//   defect / error        CPU_PoissonSolver_MG.cpp:460-514
+//   restriction           CPU_PoissonSolver_MG.cpp:533-616
+//   prolongation          CPU_PoissonSolver_MG.cpp:635-684
+//   window                CPU_PoissonSolver_MG.cpp:348-358
+//
+// Level-0 smoothing enumerates the cells of one colour directly (x = 2h + 1 + parity offset) so that no
+// lane idles; coarser levels (at most 9^3) use a plain enumeration with the wrong colour masked.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "pg_geometry.h"
+#include "pg_kernels.h"
+#include "pg_interp.h"
+
+#pragma clang fp contract(off)
+
+namespace pgamd {
+
+// Level schedule: N_{l+1} = (N_l + 1)/2 while that is >= 3 (at most 4 levels for N <= 18).
+struct MgLevels {
+  int n[4];
+  int nlv;
+  int nbottom;
+};
+
+constexpr MgLevels mg_levels_of(int n0) {
+  MgLevels L{{n0, 0, 0, 0}, 1, 0};
+  while ((L.n[L.nlv - 1] + 1) / 2 >= 3 && L.nlv < 4) {
+    L.n[L.nlv] = (L.n[L.nlv - 1] + 1) / 2;
+    L.nlv++;
+  }
+  L.nbottom = L.n[L.nlv - 1] == 3 ? 1 : 7;
+  return L;
+}
+
+template <int GS>
+struct MgGeo {
+  static constexpr Geo g = geo_of(GS);
+  static constexpr int N = g.N, R = g.R, C = g.C, U = g.U;
+  static constexpr MgLevels L = mg_levels_of(N);
+  static constexpr int NLV = L.nlv;
+  static constexpr int n(int l) { return L.n[l]; }
+  static constexpr int n3(int l) { return L.n[l] * L.n[l] * L.n[l]; }
+  // LDS layout (floats): Sol0, Def0, then Sol/RHS/Def of each coarser level, then 8 reduction slots.
+  static constexpr int off_sol(int l) { return l == 0 ? 0 : 2 * n3(0) + 3 * prefix(l); }
+  static constexpr int off_rhs(int l) { return 2 * n3(0) + 3 * prefix(l) + n3(l); }
+  static constexpr int off_def(int l) { return l == 0 ? n3(0) : 2 * n3(0) + 3 * prefix(l) + 2 * n3(l); }
+  static constexpr int prefix(int l) {  // sum of n3 over levels 1..l-1
+    int s = 0;
+    for (int k = 1; k < l; k++) s += n3(k);
+    return s;
+  }
+  static constexpr int off_red() { return 2 * n3(0) + 3 * prefix(NLV); }
+  static constexpr int FLOATS = off_red() + 8;
+  static constexpr int CELLS0 = R * R * R / 2;            // level-0 cells per colour
+  static constexpr int PER_THREAD = (CELLS0 + 255) / 256;  // level-0 cells per colour per thread
+};
+
+constexpr int kMgThreads = 256;
+
+__device__ __forceinline__ int nat(int n, int x, int y, int z) { return (z * n + y) * n + x; }
+
+// Block-wide sums of two values in a fixed order (bitwise reproducible for a given launch shape).
+__device__ __forceinline__ void block_sum2(float& a, float& b, float* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = a;
+    red[4 + w] = b;
+  }
+  __syncthreads();
+  a = ((red[0] + red[1]) + red[2]) + red[3];
+  b = ((red[4] + red[5]) + red[6]) + red[7];
+  __syncthreads();
+}
+
+// Level-0 cell q of colour c (cube parity) in the natural layout.
+template <int GS>
+__device__ __forceinline__ void cell0(int q, int c, int& x, int& y, int& z) {
+  constexpr int R = MgGeo<GS>::R, H = R / 2;
+  const int h = q % H;
+  y = (q / H) % R + 1;
+  z = q / (H * R) + 1;
+  x = 2 * h + 1 + ((c + y + z + 1) & 1);
+}
+
+// Red-black Gauss-Seidel pass of one colour at level l (CPU_PoissonSolver_MG.cpp:392-441).
+template <int GS>
+__device__ void smooth_level0(float* __restrict__ sol, const float (&rhs)[2][MgGeo<GS>::PER_THREAD], float dh2) {
+  using M = MgGeo<GS>;
+  constexpr int N = M::N;
+  const float one_six = 1.0f / 6.0f;
+#pragma unroll
+  for (int pass = 0; pass < 2; pass++) {
+    const int c = 1 - pass;
+#pragma unroll
+    for (int s = 0; s < M::PER_THREAD; s++) {
+      const int q = threadIdx.x + kMgThreads * s;
+      if (q < M::CELLS0) {
+        int x, y, z;
+        cell0<GS>(q, c, x, y, z);
+        const int id = nat(N, x, y, z);
+        sol[id] = one_six * (sol[id + N * N] + sol[id - N * N] + sol[id + N] + sol[id - N] + sol[id + 1] + sol[id - 1] -
+                             dh2 * rhs[c][s]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ void smooth_coarse(float* __restrict__ sol, const float* __restrict__ rhs, float dh2, int n) {
+  const float one_six = 1.0f / 6.0f;
+  const int m = n - 2;
+  for (int pass = 0; pass < 2; pass++) {
+    const int c = 1 - pass;
+    for (int q = threadIdx.x; q < m * m * m; q += kMgThreads) {
+      const int x = q % m + 1, y = (q / m) % m + 1, z = q / (m * m) + 1;
+      if (((x + y + z) & 1) != c) continue;
+      const int id = nat(n, x, y, z);
+      sol[id] = one_six * (sol[id + n * n] + sol[id - n * n] + sol[id + n] + sol[id - n] + sol[id + 1] + sol[id - 1] -
+                           dh2 * rhs[id]);
+    }
+    __syncthreads();
+  }
+}
+
+// Defect -(lap(sol) - rhs) on the level-0 interior, with the L1 error estimate when requested
+// (CPU_PoissonSolver_MG.cpp:460-514).
+template <int GS>
+__device__ float defect_level0(const float* __restrict__ sol, float* __restrict__ def,
+                               const float (&rhs)[2][MgGeo<GS>::PER_THREAD], float dh, bool estimate, float* red) {
+  using M = MgGeo<GS>;
+  constexpr int N = M::N;
+  const float m_dh2 = -1.0f / (dh * dh);
+  float sd = 0.0f, ss = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int s = 0; s < M::PER_THREAD; s++) {
+      const int q = threadIdx.x + kMgThreads * s;
+      if (q < M::CELLS0) {
+        int x, y, z;
+        cell0<GS>(q, c, x, y, z);
+        const int id = nat(N, x, y, z);
+        const float d = m_dh2 * (sol[id + N * N] + sol[id - N * N] + sol[id + N] + sol[id - N] + sol[id + 1] +
+                                 sol[id - 1] - 6.0f * sol[id]) +
+                        rhs[c][s];
+        def[id] = d;
+        sd += fabsf(d);
+        ss += fabsf(sol[id]);
+      }
+    }
+  __syncthreads();
+  if (!estimate) return 0.0f;
+  block_sum2(sd, ss, red);
+  return dh * dh * sd / ss;
+}
+
+__device__ void defect_coarse(const float* __restrict__ sol, const float* __restrict__ rhs, float* __restrict__ def,
+                              float dh, int n) {
+  const float m_dh2 = -1.0f / (dh * dh);
+  const int m = n - 2;
+  for (int q = threadIdx.x; q < m * m * m; q += kMgThreads) {
+    const int x = q % m + 1, y = (q / m) % m + 1, z = q / (m * m) + 1;
+    const int id = nat(n, x, y, z);
+    def[id] = m_dh2 * (sol[id + n * n] + sol[id - n * n] + sol[id + n] + sol[id - n] + sol[id + 1] + sol[id - 1] -
+                       6.0f * sol[id]) +
+              rhs[id];
+  }
+  __syncthreads();
+}
+
+// 27-point restriction with ratio (nf-1)/(nc-1) (CPU_PoissonSolver_MG.cpp:533-616).
+__device__ void restrict_level(const float* __restrict__ f, float* __restrict__ c, int nf, int nc) {
+  const float ratio = float(nf - 1) / float(nc - 1);
+  const int m = nc - 2;
+  for (int q = threadIdx.x; q < m * m * m; q += kMgThreads) {
+    const int i = q % m + 1, j = (q / m) % m + 1, k = q / (m * m) + 1;
+    float w[3][3];
+    int ctr[3];
+    const int idx[3] = {k, j, i};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const float x = float(idx[a]) * ratio;
+      ctr[a] = int(x + 0.5f);
+      w[a][0] = 0.5f * (float(ctr[a]) + 0.5f - x);
+      w[a][2] = 0.5f - w[a][0];
+      w[a][1] = 0.5f;
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          s = s + w[0][a] * w[1][b] * w[2][d] * f[nat(nf, ctr[2] - 1 + d, ctr[1] - 1 + b, ctr[0] - 1 + a)];
+    c[nat(nc, i, j, k)] = s;
+  }
+  __syncthreads();
+}
+
+// Trilinear prolongation added to the fine interior; terms in the order 000,001,010,100,011,110,101,111
+// of (z,y,x) (CPU_PoissonSolver_MG.cpp:635-684).
+__device__ void prolongate_add(const float* __restrict__ c, float* __restrict__ f, int nc, int nf) {
+  const float ratio = float(nc - 1) / float(nf - 1);
+  const int m = nf - 2;
+  const int order[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {0, 1, 1}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
+  for (int q = threadIdx.x; q < m * m * m; q += kMgThreads) {
+    const int i = q % m + 1, j = (q / m) % m + 1, k = q / (m * m) + 1;
+    float w[3][2];
+    int lo[3];
+    const int idx[3] = {k, j, i};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const float x = float(idx[a]) * ratio;
+      lo[a] = int(x);
+      w[a][0] = float(lo[a] + 1) - x;
+      w[a][1] = 1.0f - w[a][0];
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int a = order[t][0], b = order[t][1], d = order[t][2];
+      s = s + w[0][a] * w[1][b] * w[2][d] * c[nat(nc, lo[2] + d, lo[1] + b, lo[0] + a)];
+    }
+    const int id = nat(nf, i, j, k);
+    f[id] = f[id] + s;
+  }
+  __syncthreads();
+}
+
+// One patch: interpolation, V-cycles until Error <= tol or max_iter, window and optional gravity.
+template <int GS>
+__device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho, const float* __restrict__ cpot,
+                         const float* __restrict__ guess, float* __restrict__ out, float* __restrict__ flu,
+                         int* iters, int* status, const SolveParams& P) {
+  using M = MgGeo<GS>;
+  constexpr int N = M::N, R = M::R, C = M::C, U = M::U, NLV = M::NLV;
+  float* sol0 = lds + M::off_sol(0);
+  float* red = lds + M::off_red();
+
+  // interpolation into Sol0 (natural layout) and Def boundaries = 0
+  for (int t = threadIdx.x; t < M::off_red(); t += kMgThreads) lds[t] = 0.0f;
+  __syncthreads();
+  {
+    constexpr int NC = C - 2;
+    for (int q = threadIdx.x; q < NC * NC * NC; q += kMgThreads) {
+      const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
+      float v[3][3][3];
+#pragma unroll
+      for (int dk = 0; dk < 3; dk++)
+#pragma unroll
+        for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+          for (int di = 0; di < 3; di++) v[dk][dj][di] = cpot[((k + dk - 1) * C + (j + dj - 1)) * C + (i + di - 1)];
+      float ch[2][2][2];
+      interp_children(v, P.int_scheme, ch);
+#pragma unroll
+      for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+        for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+          for (int cx = 0; cx < 2; cx++) {
+            const int x = 2 * (i - 1) + cx - U, y = 2 * (j - 1) + cy - U, z = 2 * (k - 1) + cz - U;
+            if (x >= 0 && x < N && y >= 0 && y < N && z >= 0 && z < N) sol0[nat(N, x, y, z)] = ch[cz][cy][cx];
+          }
+    }
+  }
+  __syncthreads();
+
+  // level-0 right-hand side in registers; initial-guess modes
+  float rhs0[2][M::PER_THREAD];
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int s = 0; s < M::PER_THREAD; s++) {
+      const int q = threadIdx.x + kMgThreads * s;
+      rhs0[c][s] = 0.0f;
+      if (q < M::CELLS0) {
+        int x, y, z;
+        cell0<GS>(q, c, x, y, z);
+        const int gi = ((z - 1) * R + (y - 1)) * R + (x - 1);
+        rhs0[c][s] = P.poi_coeff * rho[gi];  // CPU_PoissonSolver_MG.cpp:200-207
+        if (P.init_mode == 1) sol0[nat(N, x, y, z)] = 0.0f;
+        if (P.init_mode == 2) sol0[nat(N, x, y, z)] = guess[gi];
+      }
+    }
+  __syncthreads();
+
+  float dh[4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) dh[l] = l == 0 ? P.dh : P.dh * float(N - 1) / float(l < NLV ? M::n(l) - 1 : 1);
+
+  int iter = 0;
+  float error = FLT_MAX;
+  while (iter < P.mg_max_iter && error > P.mg_tol) {
+    // down: pre-smooth, defect, restrict, zero the next correction
+#pragma unroll
+    for (int l = 0; l < NLV - 1; l++) {
+      const int n = M::n(l);
+      float* sol = lds + M::off_sol(l);
+      float* def = lds + M::off_def(l);
+      if (l == 0) {
+        for (int s = 0; s < P.mg_npre; s++) smooth_level0<GS>(sol, rhs0, dh[0] * dh[0]);
+        defect_level0<GS>(sol, def, rhs0, dh[0], false, red);
+      } else {
+        for (int s = 0; s < P.mg_npre; s++) smooth_coarse(sol, lds + M::off_rhs(l), dh[l] * dh[l], n);
+        defect_coarse(sol, lds + M::off_rhs(l), def, dh[l], n);
+      }
+      restrict_level(def, lds + M::off_rhs(l + 1), n, M::n(l + 1));
+      float* nsol = lds + M::off_sol(l + 1);
+      for (int t = threadIdx.x; t < M::n3(l + 1); t += kMgThreads) nsol[t] = 0.0f;
+      __syncthreads();
+    }
+    // bottom
+    {
+      constexpr int b = NLV - 1;
+      for (int s = 0; s < M::L.nbottom; s++) smooth_coarse(lds + M::off_sol(b), lds + M::off_rhs(b), dh[b] * dh[b], M::n(b));
+    }
+    // up: prolongate and correct, post-smooth
+#pragma unroll
+    for (int l = NLV - 2; l >= 0; l--) {
+      prolongate_add(lds + M::off_sol(l + 1), lds + M::off_sol(l), M::n(l + 1), M::n(l));
+      if (l == 0) {
+        for (int s = 0; s < P.mg_npost; s++) smooth_level0<GS>(sol0, rhs0, dh[0] * dh[0]);
+      } else {
+        for (int s = 0; s < P.mg_npost; s++)
+          smooth_coarse(lds + M::off_sol(l), lds + M::off_rhs(l), dh[l] * dh[l], M::n(l));
+      }
+    }
+    error = defect_level0<GS>(sol0, lds + M::off_def(0), rhs0, dh[0], true, red);
+    iter++;
+  }
+
+  // status: converged / max_iter / non-finite
+  if (threadIdx.x == 0) {
+    int st = error <= P.mg_tol ? kStatusConverged : kStatusMaxIter;
+    if (!(error == error) || error == INFINITY) st |= kStatusNonFinite;
+    if (iters) *iters = iter;
+    if (status) *status = st;
+  }
+
+  // window (CPU_PoissonSolver_MG.cpp:348-358) and fused gravity
+  const int G = gra_nxt(P.gra_ghost), o = GS - P.gra_ghost;
+  for (int t = threadIdx.x; t < G * G * G; t += kMgThreads) {
+    const int i = t % G, j = (t / G) % G, k = t / (G * G);
+    out[t] = sol0[nat(N, i + o, j + o, k + o)];
+  }
+  if (P.fuse_gravity && flu) {
+    for (int t = threadIdx.x; t < kPatchCells; t += kMgThreads) {
+      const int i = t % kPatchSize, j = (t / kPatchSize) % kPatchSize, k = t / (kPatchSize * kPatchSize);
+      const int x = i + GS, y = j + GS, z = k + GS;
+      auto Q = [&](int dz, int dy, int dx) { return sol0[nat(N, x + dx, y + dy, z + dz)]; };
+      float ax, ay, az;
+      if (P.p5) {
+        ax = P.gra_const * (-Q(0, 0, 2) + Q(0, 0, -2) + 8.0f * Q(0, 0, 1) - 8.0f * Q(0, 0, -1));
+        ay = P.gra_const * (-Q(0, 2, 0) + Q(0, -2, 0) + 8.0f * Q(0, 1, 0) - 8.0f * Q(0, -1, 0));
+        az = P.gra_const * (-Q(2, 0, 0) + Q(-2, 0, 0) + 8.0f * Q(1, 0, 0) - 8.0f * Q(-1, 0, 0));
+      } else {
+        ax = P.gra_const * (Q(0, 0, 1) - Q(0, 0, -1));
+        ay = P.gra_const * (Q(0, 1, 0) - Q(0, -1, 0));
+        az = P.gra_const * (Q(1, 0, 0) - Q(-1, 0, 0));
+      }
+      const float rho_f = flu[t];
+      float px = flu[kPatchCells + t], py = flu[2 * kPatchCells + t], pz = flu[3 * kPatchCells + t];
+      const float temp = 0.5f / rho_f;
+      const float eint = flu[4 * kPatchCells + t] - temp * (px * px + py * py + pz * pz);
+      px = px - rho_f * ax;
+      py = py - rho_f * ay;
+      pz = pz - rho_f * az;
+      flu[kPatchCells + t] = px;
+      flu[2 * kPatchCells + t] = py;
+      flu[3 * kPatchCells + t] = pz;
+      flu[4 * kPatchCells + t] = eint + temp * (px * px + py * py + pz * pz);
+    }
+  }
+  __syncthreads();
+}
+
+template <int GS>
+__global__ void __launch_bounds__(kMgThreads) mg_uniform_kernel(UniformBatch b, SolveParams P) {
+  extern __shared__ float lds[];
+  using M = MgGeo<GS>;
+  constexpr int R3 = M::R * M::R * M::R, C3 = M::C * M::C * M::C;
+  const int p = blockIdx.x;
+  if (p >= b.npatch) return;
+  const int G = gra_nxt(P.gra_ghost);
+  mg_patch<GS>(lds, b.rho + (size_t)p * R3, b.cpot + (size_t)p * C3, b.guess ? b.guess + (size_t)p * R3 : nullptr,
+               b.pot_out + (size_t)p * G * G * G, b.flu ? b.flu + (size_t)p * kNComp * kPatchCells : nullptr,
+               b.iters ? b.iters + p : nullptr, b.status ? b.status + p : nullptr, P);
+}
+
+size_t lds_bytes_mg(int gs) {
+  switch (gs) {
+    case 1: return sizeof(float) * MgGeo<1>::FLOATS;
+    case 2: return sizeof(float) * MgGeo<2>::FLOATS;
+    case 3: return sizeof(float) * MgGeo<3>::FLOATS;
+    case 4: return sizeof(float) * MgGeo<4>::FLOATS;
+    case 5: return sizeof(float) * MgGeo<5>::FLOATS;
+  }
+  return 0;
+}
+
+template <int GS>
+static hipError_t launch_mg(const UniformBatch& b, const SolveParams& p, hipStream_t s) {
+  const size_t lds = lds_bytes_mg(GS);
+  hipLaunchKernelGGL((mg_uniform_kernel<GS>), dim3(b.npatch), dim3(kMgThreads), lds, s, b, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_mg_uniform(const UniformBatch& b, const SolveParams& p, hipStream_t s) {
+  switch (b.gs) {
+    case 1: return launch_mg<1>(b, p, s);
+    case 2: return launch_mg<2>(b, p, s);
+    case 3: return launch_mg<3>(b, p, s);
+    case 4: return launch_mg<4>(b, p, s);
+    case 5: return launch_mg<5>(b, p, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pgamd

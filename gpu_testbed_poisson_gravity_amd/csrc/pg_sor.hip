@@ -1,0 +1,457 @@
+// Red-black SOR on gfx950 with the whole solve cube resident in LDS, one wavefront per patch (or per 2-4
+// small patches), plus the coarse-to-fine interpolation, the output window and the fused gravity update.
+//
+// Reference behaviour reproduced here (bit for bit in FP32 when the same iteration count is run):
+//   interpolation  CPU_PoissonSolver_SOR.cpp:110-203   (GPU: CUPOT_PoissonSolver_SOR_10to14cube.cu:185-343)
+//   SOR sweep      CPU_PoissonSolver_SOR.cpp:215-270   (GPU: CUPOT_PoissonSolver_SOR_10to14cube.cu:349-405)
+//   stall rule     CPU_PoissonSolver_SOR.cpp:273-283
+//   window         CPU_PoissonSolver_SOR.cpp:303-311
+//   gravity        CPU_GravitySolver.cpp:34-79         (GPU: CUPOT_GravitySolver.cu:24-115)
+//
+// Layout.  The N^3 cube lives in LDS split by colour: cell (x,y,z) is stored at
+//     lds[((c*N + z)*N + y)*HP + (x>>1)],  c = (x+y+z)&1,  HP = N/2,
+// so the cells of one colour along a row are contiguous.  A lane owns a 2x2 block of z-columns
+// (x in {2m+1, 2m+2}, y in {2yb+1, 2yb+2}) over the whole interior height and keeps those cells' phi and
+// Const*rho in registers.  Updating one cell then needs only two LDS reads (the neighbour across the
+// x-pair and the one across the y-pair, both at row offset m+xx in the other colour's array) and one
+// LDS write; z-neighbours and the other half of the x/y neighbours come from the lane's own registers.
+// Each patch needs (R/2)^2 lanes: 16, 25, 36, 49, 64 for ghost sizes 1..5, so 4 (gs 1) or 2 (gs 2)
+// patches can share a wavefront.  One workgroup is one wavefront; barriers are single-wave barriers.
+//
+// Relative-residual rule (no extra sweep): at a check the first-colour residuals are the pre-update
+// residuals of the pass being run, the second-colour ones are |1-omega| times the pre-update residuals
+// of the previous second pass; the pass is only committed if the patch has not converged.
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "pg_geometry.h"
+#include "pg_kernels.h"
+#include "pg_interp.h"
+#include "pg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace pgamd {
+
+template <int GS>
+struct SorGeo {
+  static constexpr Geo g = geo_of(GS);
+  static constexpr int N = g.N;
+  static constexpr int R = g.R;
+  static constexpr int C = g.C;
+  static constexpr int U = g.U;
+  static constexpr int HP = N / 2;              // split row length in floats
+  static constexpr int MX = R / 2;              // x-pairs per row
+  static constexpr int LP = MX * (R / 2);       // lanes per patch
+  static constexpr int FLOATS = 2 * N * N * HP; // == N^3
+};
+
+__device__ __forceinline__ int cube_index(int N, int HP, int x, int y, int z) {
+  return ((((x + y + z) & 1) * N + z) * N + y) * HP + (x >> 1);
+}
+
+// Sum over a power-of-two lane segment; every lane of the segment gets a bitwise-identical result
+// (each butterfly step adds the same two operands, only commuted).
+template <int SEG>
+__device__ __forceinline__ float seg_sum(float v) {
+#pragma unroll
+  for (int off = SEG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ---- interpolation into LDS --------------------------------------------------------------------------
+
+// Interpolate a patch's coarse potential into its LDS cube (split layout).  `lane`/`nlanes` index the
+// lanes serving this patch.
+template <int GS>
+__device__ void interp_to_lds(const float* __restrict__ cp, float* __restrict__ lds, int lane, int nlanes,
+                              int int_scheme) {
+  using S = SorGeo<GS>;
+  constexpr int C = S::C, N = S::N, U = S::U, HP = S::HP, NC = C - 2;
+  for (int q = lane; q < NC * NC * NC; q += nlanes) {
+    const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
+    float v[3][3][3];
+#pragma unroll
+    for (int dk = 0; dk < 3; dk++)
+#pragma unroll
+      for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+        for (int di = 0; di < 3; di++) v[dk][dj][di] = cp[((k + dk - 1) * C + (j + dj - 1)) * C + (i + di - 1)];
+    float ch[2][2][2];
+    interp_children(v, int_scheme, ch);
+#pragma unroll
+    for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+      for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+        for (int cx = 0; cx < 2; cx++) {
+          const int x = 2 * (i - 1) + cx - U, y = 2 * (j - 1) + cy - U, z = 2 * (k - 1) + cz - U;
+          if (x >= 0 && x < N && y >= 0 && y < N && z >= 0 && z < N) lds[cube_index(N, HP, x, y, z)] = ch[cz][cy][cx];
+        }
+  }
+}
+
+// ---- per-lane state and sweeps -----------------------------------------------------------------------
+
+template <int GS>
+struct LaneCells {
+  static constexpr int R = SorGeo<GS>::R;
+  float phi[2][2][R];  // [yy][xx][z-1]
+  float cr[2][2][R];   // Const*rho
+  float bz[2][2][2];   // Dirichlet values below/above each owned column
+};
+
+// LDS index of cell (x = 2m+1+xx, y = 2yb+1+yy, z) relative to the lane base lb = 2*yb*HP + m.
+template <int GS>
+__device__ __forceinline__ constexpr int own_off(int c, int z, int yy, int xx) {
+  using S = SorGeo<GS>;
+  return ((c * S::N + z) * S::N + (yy + 1)) * S::HP + xx;
+}
+
+// Pre-update residuals of the lane's cells of colour COL (cube parity), reference operand order
+// (CPU_PoissonSolver_SOR.cpp:247-250): ((((((kp+km)+jp)+jm)+ip)+im) - 6 phi) - Const*rho.
+template <int GS, int COL>
+__device__ __forceinline__ void residuals(const LaneCells<GS>& L, const float* __restrict__ lds, int lb,
+                                          float (&res)[2][SorGeo<GS>::R]) {
+  constexpr int R = SorGeo<GS>::R;
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int zi = 0; zi < R; zi++) {
+      const int xx = (COL + yy + zi + 1) & 1;
+      const int z = zi + 1;
+      const float kp = zi + 1 < R ? L.phi[yy][xx][zi + 1] : L.bz[yy][xx][1];
+      const float km = zi > 0 ? L.phi[yy][xx][zi - 1] : L.bz[yy][xx][0];
+      const float jp = yy == 0 ? L.phi[1][xx][zi] : lds[lb + own_off<GS>(1 - COL, z, 2, xx)];
+      const float jm = yy == 1 ? L.phi[0][xx][zi] : lds[lb + own_off<GS>(1 - COL, z, -1, xx)];
+      const float ip = xx == 0 ? L.phi[yy][1][zi] : lds[lb + own_off<GS>(1 - COL, z, yy, xx)];
+      const float im = xx == 1 ? L.phi[yy][0][zi] : lds[lb + own_off<GS>(1 - COL, z, yy, xx)];
+      res[yy][zi] = kp + km + jp + jm + ip + im - 6.0f * L.phi[yy][xx][zi] - L.cr[yy][xx][zi];
+    }
+}
+
+template <int GS, int COL>
+__device__ __forceinline__ void apply(LaneCells<GS>& L, float* __restrict__ lds, int lb,
+                                      const float (&res)[2][SorGeo<GS>::R], float w6) {
+  constexpr int R = SorGeo<GS>::R;
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int zi = 0; zi < R; zi++) {
+      const int xx = (COL + yy + zi + 1) & 1;
+      const float v = L.phi[yy][xx][zi] + w6 * res[yy][zi];
+      L.phi[yy][xx][zi] = v;
+      lds[lb + own_off<GS>(COL, zi + 1, yy, xx)] = v;
+    }
+}
+
+template <int GS>
+__device__ __forceinline__ float abs_sum(const float (&res)[2][SorGeo<GS>::R]) {
+  float s = 0.0f;
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int zi = 0; zi < SorGeo<GS>::R; zi++) s += fabsf(res[yy][zi]);
+  return s;
+}
+
+// Per-lane view of the patch a slot of the wavefront works on.
+struct LanePatch {
+  const float* rho;
+  const float* cpot;
+  const float* guess;
+  float* out;
+  float* flu;
+  int* iters;
+  int* status;
+  bool valid;
+};
+
+// Gravity update of one patch from the potential held in an LDS cube (CPU_GravitySolver.cpp:34-79).
+template <int GS>
+__device__ void gravity_from_lds(const float* __restrict__ lds, float* __restrict__ flu, int lane, int nlanes,
+                                 float gra_const, int p5) {
+  using S = SorGeo<GS>;
+  constexpr int N = S::N, HP = S::HP;
+  for (int t = lane; t < kPatchCells; t += nlanes) {
+    const int i = t % kPatchSize, j = (t / kPatchSize) % kPatchSize, k = t / (kPatchSize * kPatchSize);
+    const int x = i + GS, y = j + GS, z = k + GS;
+    auto P = [&](int dz, int dy, int dx) { return lds[cube_index(N, HP, x + dx, y + dy, z + dz)]; };
+    float ax, ay, az;
+    if (p5) {
+      ax = gra_const * (-P(0, 0, 2) + P(0, 0, -2) + 8.0f * P(0, 0, 1) - 8.0f * P(0, 0, -1));
+      ay = gra_const * (-P(0, 2, 0) + P(0, -2, 0) + 8.0f * P(0, 1, 0) - 8.0f * P(0, -1, 0));
+      az = gra_const * (-P(2, 0, 0) + P(-2, 0, 0) + 8.0f * P(1, 0, 0) - 8.0f * P(-1, 0, 0));
+    } else {
+      ax = gra_const * (P(0, 0, 1) - P(0, 0, -1));
+      ay = gra_const * (P(0, 1, 0) - P(0, -1, 0));
+      az = gra_const * (P(1, 0, 0) - P(-1, 0, 0));
+    }
+    const float rho = flu[t];
+    float px = flu[kPatchCells + t], py = flu[2 * kPatchCells + t], pz = flu[3 * kPatchCells + t];
+    const float temp = 0.5f / rho;
+    const float eint = flu[4 * kPatchCells + t] - temp * (px * px + py * py + pz * pz);
+    px = px - rho * ax;
+    py = py - rho * ay;
+    pz = pz - rho * az;
+    flu[kPatchCells + t] = px;
+    flu[2 * kPatchCells + t] = py;
+    flu[3 * kPatchCells + t] = pz;
+    flu[4 * kPatchCells + t] = eint + temp * (px * px + py * py + pz * pz);
+  }
+}
+
+// Solve up to PPW patches of ghost size GS with one wavefront (blockDim.x == 64).
+template <int GS, int PPW>
+__device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, const SolveParams& P) {
+  using S = SorGeo<GS>;
+  constexpr int N = S::N, R = S::R, HP = S::HP, MX = S::MX, LP = S::LP;
+  constexpr int SEG = 64 / PPW;
+  static_assert(LP <= SEG, "patch does not fit its lane segment");
+  const int lane = threadIdx.x;
+  const int slot = lane / SEG, pl = lane % SEG;
+  float* lds = lds_wave + slot * S::FLOATS;
+  const bool active = lp.valid && pl < LP;
+  const int plc = pl < LP ? pl : LP - 1;  // idle lanes mirror a real lane's addresses and never write
+  const int m = plc % MX, yb = plc / MX;
+  const int lb = 2 * yb * HP + m;
+
+  // A. coarse -> fine interpolation of the whole cube (initial guess and Dirichlet layer)
+  if (lp.valid) interp_to_lds<GS>(lp.cpot, lds, pl, SEG, P.int_scheme);
+  __syncthreads();
+
+  // B. owned cells into registers
+  LaneCells<GS> L;
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int xx = 0; xx < 2; xx++) {
+      L.bz[yy][xx][0] = lds[lb + own_off<GS>((xx + yy) & 1, 0, yy, xx)];
+      L.bz[yy][xx][1] = lds[lb + own_off<GS>((xx + yy + N - 1) & 1, N - 1, yy, xx)];
+#pragma unroll
+      for (int zi = 0; zi < R; zi++) {
+        const int c = (xx + yy + zi + 1) & 1;
+        const int gi = (zi * R + 2 * yb + yy) * R + 2 * m + xx;
+        float v;
+        if (P.init_mode == 0)
+          v = lds[lb + own_off<GS>(c, zi + 1, yy, xx)];
+        else if (P.init_mode == 1)
+          v = 0.0f;
+        else
+          v = active ? lp.guess[gi] : 0.0f;
+        L.phi[yy][xx][zi] = v;
+        L.cr[yy][xx][zi] = active ? P.cst * lp.rho[gi] : 0.0f;
+        if (P.init_mode != 0 && active) lds[lb + own_off<GS>(c, zi + 1, yy, xx)] = v;
+      }
+    }
+  __syncthreads();
+
+  // C. red-black iterations
+  bool running = active;
+  int it_done = P.max_iter, st = kStatusMaxIter;
+  const bool relres = P.term_mode == 1, stall = P.term_mode == 0;
+  const int first_check = P.min_iter > 1 ? P.min_iter : 1;
+  float sum_b = 0.0f, old = FLT_MAX;
+  float res[2][R];
+  for (int t = 0; t < P.max_iter; ++t) {
+    residuals<GS, 1>(L, lds, lb, res);
+    float tot = 0.0f;
+    if (relres && t >= first_check && (t - first_check) % P.check_every == 0) {
+      float num = abs_sum<GS>(res), den = 0.0f;
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+          for (int zi = 0; zi < R; zi++) den += fabsf(L.phi[yy][xx][zi]);
+      num = num + P.damp * sum_b;
+      if (!active) num = den = 0.0f;
+      num = seg_sum<SEG>(num);
+      den = seg_sum<SEG>(den);
+      if (running && num <= P.rel_tol * den) {
+        running = false;
+        it_done = t;
+        st = kStatusConverged;
+      }
+      if (!__syncthreads_or(running)) break;
+    } else if (stall) {
+      tot = abs_sum<GS>(res);
+    }
+    if (running) apply<GS, 1>(L, lds, lb, res, P.w6);
+    __syncthreads();
+    residuals<GS, 0>(L, lds, lb, res);
+    if (relres && t + 1 >= first_check && (t + 1 - first_check) % P.check_every == 0) sum_b = abs_sum<GS>(res);
+    if (stall) tot += abs_sum<GS>(res);
+    if (running) apply<GS, 0>(L, lds, lb, res, P.w6);
+    __syncthreads();
+    if (stall) {
+      tot = seg_sum<SEG>(active ? tot : 0.0f);
+      if (running && t + 1 >= P.min_iter && tot > old) {
+        running = false;
+        it_done = t + 1;
+        st = kStatusConverged;
+      }
+      old = tot;
+      if (!__syncthreads_or(running)) break;
+    }
+  }
+
+  // non-finite check over the owned cells of each patch
+  bool bad = false;
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+      for (int zi = 0; zi < R; zi++) bad |= !isfinite(L.phi[yy][xx][zi]);
+  const uint64_t bad_mask = __ballot(active && bad);
+  const uint64_t seg_mask = SEG == 64 ? ~0ull : (((1ull << SEG) - 1) << (slot * SEG));
+  if (bad_mask & seg_mask) st |= kStatusNonFinite;
+  if (lp.valid && pl == 0) {
+    if (lp.iters) *lp.iters = it_done;
+    if (lp.status) *lp.status = st;
+  }
+  __syncthreads();
+
+  // D. output window: out[k][j][i] = cube[k+o][j+o][i+o], o = GS - GRA_GHOST_SIZE
+  if (lp.valid) {
+    const int G = gra_nxt(P.gra_ghost), o = GS - P.gra_ghost;
+    for (int t = pl; t < G * G * G; t += SEG) {
+      const int i = t % G, j = (t / G) % G, k = t / (G * G);
+      lp.out[t] = lds[cube_index(N, HP, i + o, j + o, k + o)];
+    }
+    if (P.fuse_gravity && lp.flu) gravity_from_lds<GS>(lds, lp.flu, pl, SEG, P.gra_const, P.p5);
+  }
+  __syncthreads();
+}
+
+// ---- kernels -----------------------------------------------------------------------------------------
+
+template <int GS>
+__device__ __forceinline__ LanePatch uniform_patch(const UniformBatch& b, int patch) {
+  using S = SorGeo<GS>;
+  constexpr int R3 = S::R * S::R * S::R, C3 = S::C * S::C * S::C;
+  LanePatch lp;
+  lp.valid = patch < b.npatch;
+  const int p = lp.valid ? patch : 0;
+  lp.rho = b.rho + (size_t)p * R3;
+  lp.cpot = b.cpot + (size_t)p * C3;
+  lp.guess = b.guess ? b.guess + (size_t)p * R3 : nullptr;
+  lp.out = nullptr;  // set by the caller (depends on the window size)
+  lp.flu = b.flu ? b.flu + (size_t)p * kNComp * kPatchCells : nullptr;
+  lp.iters = b.iters ? b.iters + p : nullptr;
+  lp.status = b.status ? b.status + p : nullptr;
+  return lp;
+}
+
+template <int GS, int PPW>
+__global__ void __launch_bounds__(64) sor_uniform_kernel(UniformBatch b, SolveParams P) {
+  extern __shared__ float lds[];
+  constexpr int SEG = 64 / PPW;
+  const int slot = threadIdx.x / SEG;
+  const int patch = blockIdx.x * PPW + slot;
+  LanePatch lp = uniform_patch<GS>(b, patch);
+  const int G = gra_nxt(P.gra_ghost);
+  lp.out = b.pot_out + (size_t)(lp.valid ? patch : 0) * G * G * G;
+  sor_task<GS, PPW>(lds, lp, P);
+}
+
+// Standalone gravity kernel for the Poisson=false path (CUAPI_Asyn_PoissonGravitySolver.cu:192-200,
+// 241-246): one workgroup of 256 threads per patch, potential read from HBM.
+__global__ void __launch_bounds__(256) gravity_kernel(const float* __restrict__ pot, float* __restrict__ flu,
+                                                      int npatch, int gg, float gra_const, int p5) {
+  const int P = blockIdx.x;
+  if (P >= npatch) return;
+  const int G = gra_nxt(gg);
+  const float* pp = pot + (size_t)P * G * G * G;
+  float* f = flu + (size_t)P * kNComp * kPatchCells;
+  for (int t = threadIdx.x; t < kPatchCells; t += blockDim.x) {
+    const int i = t % kPatchSize, j = (t / kPatchSize) % kPatchSize, k = t / (kPatchSize * kPatchSize);
+    const int x = i + gg, y = j + gg, z = k + gg;
+    auto Q = [&](int dz, int dy, int dx) { return pp[((z + dz) * G + (y + dy)) * G + (x + dx)]; };
+    float ax, ay, az;
+    if (p5) {
+      ax = gra_const * (-Q(0, 0, 2) + Q(0, 0, -2) + 8.0f * Q(0, 0, 1) - 8.0f * Q(0, 0, -1));
+      ay = gra_const * (-Q(0, 2, 0) + Q(0, -2, 0) + 8.0f * Q(0, 1, 0) - 8.0f * Q(0, -1, 0));
+      az = gra_const * (-Q(2, 0, 0) + Q(-2, 0, 0) + 8.0f * Q(1, 0, 0) - 8.0f * Q(-1, 0, 0));
+    } else {
+      ax = gra_const * (Q(0, 0, 1) - Q(0, 0, -1));
+      ay = gra_const * (Q(0, 1, 0) - Q(0, -1, 0));
+      az = gra_const * (Q(1, 0, 0) - Q(-1, 0, 0));
+    }
+    const float rho = f[t];
+    float px = f[kPatchCells + t], py = f[2 * kPatchCells + t], pz = f[3 * kPatchCells + t];
+    const float temp = 0.5f / rho;
+    const float eint = f[4 * kPatchCells + t] - temp * (px * px + py * py + pz * pz);
+    px = px - rho * ax;
+    py = py - rho * ay;
+    pz = pz - rho * az;
+    f[kPatchCells + t] = px;
+    f[2 * kPatchCells + t] = py;
+    f[3 * kPatchCells + t] = pz;
+    f[4 * kPatchCells + t] = eint + temp * (px * px + py * py + pz * pz);
+  }
+}
+
+// ---- host launchers ------------------------------------------------------------------------------------
+
+int sor_lanes_per_patch(int gs) {
+  const int r = geo_of(gs).R;
+  return (r / 2) * (r / 2);
+}
+
+int supported_ppw(int gs, int ppw) {
+  if (ppw != 1 && ppw != 2 && ppw != 4) return 0;
+  return sor_lanes_per_patch(gs) <= 64 / ppw;
+}
+
+int default_ppw(int gs) {
+  for (int ppw : {4, 2, 1})
+    if (supported_ppw(gs, ppw)) return ppw;
+  return 1;
+}
+
+size_t lds_bytes_sor(int gs, int ppw) {
+  const Geo g = geo_of(gs);
+  return sizeof(float) * (size_t)ppw * g.N * g.N * g.N;
+}
+
+template <int GS, int PPW>
+static hipError_t launch_sor_uniform(const UniformBatch& b, const SolveParams& p, hipStream_t s) {
+  const int blocks = (b.npatch + PPW - 1) / PPW;
+  const size_t lds = lds_bytes_sor(GS, PPW);
+  hipLaunchKernelGGL((sor_uniform_kernel<GS, PPW>), dim3(blocks), dim3(64), lds, s, b, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_sor_uniform_dispatch(const UniformBatch& b, const SolveParams& p, int ppw, hipStream_t s) {
+  switch (b.gs * 8 + ppw) {
+    case 1 * 8 + 1: return launch_sor_uniform<1, 1>(b, p, s);
+    case 1 * 8 + 2: return launch_sor_uniform<1, 2>(b, p, s);
+    case 1 * 8 + 4: return launch_sor_uniform<1, 4>(b, p, s);
+    case 2 * 8 + 1: return launch_sor_uniform<2, 1>(b, p, s);
+    case 2 * 8 + 2: return launch_sor_uniform<2, 2>(b, p, s);
+    case 3 * 8 + 1: return launch_sor_uniform<3, 1>(b, p, s);
+    case 4 * 8 + 1: return launch_sor_uniform<4, 1>(b, p, s);
+    case 5 * 8 + 1: return launch_sor_uniform<5, 1>(b, p, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_uniform(const UniformBatch& b, const SolveParams& p, const SorLayout& lay, hipStream_t s) {
+  if (b.npatch <= 0) return hipSuccess;
+  if (b.scheme == kSchemeMG) return launch_mg_uniform(b, p, s);
+  return launch_sor_uniform_dispatch(b, p, lay.ppw[b.gs], s);
+}
+
+hipError_t launch_gravity(const float* pot, float* flu, int npatch, int gra_ghost, float gra_const, int p5,
+                          hipStream_t stream) {
+  if (npatch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gravity_kernel, dim3(npatch), dim3(256), 0, stream, pot, flu, npatch, gra_ghost, gra_const, p5);
+  return hipGetLastError();
+}
+
+}  // namespace pgamd

@@ -1,0 +1,134 @@
+/*
+ * pg_amd.h - C ABI of the MI355X (gfx950) batched per-patch Poisson-gravity solver.
+ *
+ * Drop-in for the GPU boundary of the reference testbed (GAMER's Poisson-gravity test program): the entry
+ * points below replace the CUAPI_* functions of GPU_API/ and keep their argument order and meaning.  All
+ * arrays are FP32, C row-major with x fastest; sizes that the reference fixes at compile time
+ * (POT_GHOST_SIZE, GRA_GHOST_SIZE, POT_SCHEME) are context options here.
+ *
+ *   rho     [npatch][R][R][R]       R = 6 + 2*POT_GHOST_SIZE          (RHO_NXT)
+ *   pot_in  [npatch][C][C][C]       C = 4 + 2*((POT_GHOST_SIZE+3)/2)  (POT_NXT)
+ *   pot_out [npatch][G][G][G]       G = 8 + 2*GRA_GHOST_SIZE          (GRA_NXT)
+ *   flu     [npatch][5][8][8][8]    (rho, px, py, pz, E)
+ *
+ * Every function returns PGAMD_OK (0) or a negative status; pgamd_last_error() describes the last failure
+ * on the calling thread.  Nothing here exits the process (the reference exits on any error,
+ * Header/CUAPI.h:10-20).  A context belongs to one host thread at a time.
+ */
+#ifndef PG_AMD_H
+#define PG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGAMD_OK 0
+#define PGAMD_ERR_ARG (-1)      /* invalid argument: null pointer, size, enum or combination          */
+#define PGAMD_ERR_DEVICE (-2)   /* HIP runtime error                                                  */
+#define PGAMD_ERR_STATE (-3)    /* call out of order (e.g. solve before allocate)                     */
+#define PGAMD_ERR_NOMEM (-4)    /* allocation failed                                                  */
+
+/* Poisson schemes and interpolation schemes keep the reference's values (Header/Macro.h:47-48,
+ * Header/Typedef.h:7-8). */
+#define PGAMD_SOR 1
+#define PGAMD_MG 2
+#define PGAMD_INT_CQUAD 4
+#define PGAMD_INT_QUAD 5
+
+/* Initial guess of the interior (the Dirichlet layer always comes from the interpolated coarse potential).
+ * The reference always starts from the interpolated cube (CPU_PoissonSolver_SOR.cpp:65). */
+#define PGAMD_INIT_INTERP 0
+#define PGAMD_INIT_ZERO 1
+#define PGAMD_INIT_GUESS 2
+
+/* SOR stopping rule.  REFERENCE: stop once the L1 residual of an iteration grows (after Min_Iter),
+ * CPU_PoissonSolver_SOR.cpp:273-283.  RELRES: stop when sum|R| <= rel_tol * sum|phi| over the interior,
+ * tested after max(1,Min_Iter) iterations and then every check_every iterations.  MG always uses the
+ * reference's relative error dh^2 sum|Def| / sum|Sol| <= MG_Tolerated_Error. */
+#define PGAMD_TERM_REFERENCE 0
+#define PGAMD_TERM_RELRES 1
+
+/* Per-patch status bits (pgamd_get_patch_stats, the *_device entries). */
+#define PGAMD_STATUS_CONVERGED 1
+#define PGAMD_STATUS_MAX_ITER 2
+#define PGAMD_STATUS_NONFINITE 4
+
+typedef struct pgamd_context pgamd_context;
+
+/* Options the reference fixes with compile-time macros (Makefile:10-53, Header/CUPOT.h). */
+typedef struct {
+  int pot_ghost_size;   /* POT_GHOST_SIZE, 1..5                                   */
+  int gra_ghost_size;   /* GRA_GHOST_SIZE, 1..2, <= pot_ghost_size                 */
+  int pot_scheme;       /* PGAMD_SOR or PGAMD_MG                                   */
+  int init_mode;        /* PGAMD_INIT_*                                            */
+  int term_mode;        /* PGAMD_TERM_*                                            */
+  int check_every;      /* >= 1                                                    */
+  float rel_tol;        /* RELRES tolerance                                        */
+  int fuse_gravity;     /* 1: gravity update inside the solve kernel, phi still in LDS */
+} pgamd_options;
+
+/* Fills the defaults: ghost 5/1, SOR, interpolated guess, reference rule, check_every 1, rel_tol 1e-5,
+ * fused gravity. */
+void pgamd_default_options(pgamd_options* opt);
+
+/* Replaces CUAPI_Set_Diagnose_Device (GPU_API/CUAPI_Set_Diagnose_Device.cu:45-292): selects the device
+ * and creates a context.  *npatch_group and *nstream receive defaults when they are <= 0 on input
+ * (reference defaults: NPGroup = 32 * number of multiprocessors, GPU_NStream = 48, :236-271; here one
+ * stream by default because a single launch already fills the chip). */
+int pgamd_set_diagnose_device(int device_id, int* npatch_group, int* nstream, pgamd_context** ctx);
+int pgamd_destroy(pgamd_context* ctx);
+int pgamd_set_options(pgamd_context* ctx, const pgamd_options* opt);
+const char* pgamd_last_error(void);
+/* Writes "<device name>;CUs=<n>;LDS/CU=<bytes>;clock_kHz=<n>" into buf. */
+int pgamd_device_info(pgamd_context* ctx, char* buf, int buflen);
+
+/* Replaces CUAPI_MemAllocate_PoissonGravity (GPU_API/CUAPI_MemAllocate_PoissonGravity.cu:27-80): device
+ * buffers for 8*npatch_group patches at the context's ghost sizes, pinned host buffers handed to the
+ * caller, and the streams. */
+int pgamd_mem_allocate_poisson_gravity(pgamd_context* ctx, int npatch_group, float** h_rho, float** h_pot_in,
+                                       float** h_pot_out, float** h_flu);
+/* Replaces CUAPI_MemFree_PoissonGravity (GPU_API/CUAPI_MemFree_PoissonGravity.cu:27-70). */
+int pgamd_mem_free_poisson_gravity(pgamd_context* ctx, float** h_rho, float** h_pot_in, float** h_pot_out,
+                                   float** h_flu);
+
+/* Replaces CUAPI_Asyn_PoissonGravitySolver (GPU_API/CUAPI_Asyn_PoissonGravitySolver.cu:85-275); same
+ * argument order.  Host arrays in (pinned ones from pgamd_mem_allocate_poisson_gravity overlap best);
+ * returns before the GPU finishes: h_pot_out and h_flu are valid after pgamd_synchronize.  With
+ * poisson == 0 and graacc != 0, h_pot_out is an input (:192-200). */
+int pgamd_asyn_poisson_gravity_solver(pgamd_context* ctx, const float* h_rho, float* h_pot_in, float* h_pot_out,
+                                      float* h_flu, int npatch_group, float dt, float dh, int sor_min_iter,
+                                      int sor_max_iter, float sor_omega, int mg_max_iter, int mg_npre_smooth,
+                                      int mg_npost_smooth, float mg_tolerated_error, float poi_coeff,
+                                      int int_scheme, int p5_gradient, int poisson, int graacc, int gpu_nstream);
+/* Replaces CUAPI_Synchronize (GPU_API/CUAPI_Synchronize.cu:10-13). */
+int pgamd_synchronize(pgamd_context* ctx);
+/* Iteration count and status of every patch of the last asynchronous solve (valid after synchronize).
+ * No reference counterpart: the reference only accumulates an average on the CPU
+ * (CPU_PoissonSolver_SOR.cpp:290). */
+int pgamd_get_patch_stats(pgamd_context* ctx, int* iters, int* status, int npatch);
+
+/* ---- device-resident entry points (inputs already in HBM; no reference counterpart: the reference
+ * always stages through host memory).  All pointers are device pointers; `stream` is a hipStream_t
+ * (NULL = the HIP default stream).  iters / status may be NULL. */
+typedef struct {
+  float dt, dh;
+  int sor_min_iter, sor_max_iter;
+  float sor_omega;
+  int mg_max_iter, mg_npre_smooth, mg_npost_smooth;
+  float mg_tolerated_error;
+  float poi_coeff;
+  int int_scheme;
+  int p5_gradient;
+  int poisson, graacc;
+} pgamd_solver_args;
+
+int pgamd_solve_device(pgamd_context* ctx, const float* d_rho, const float* d_pot_in, const float* d_guess,
+                       float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* args, int* d_iters,
+                       int* d_status, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PG_AMD_H */
