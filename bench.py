@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--check-every", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=512, help="patches timed on the CPU oracle (0: skip)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--pad", type=int, default=None, help="SOR split-row padding (default: library choice)")
+    ap.add_argument("--ppw", type=int, default=None, help="patches per wavefront (default: library choice)")
     return ap.parse_args()
 
 
@@ -70,6 +72,9 @@ def main():
     ctx = pg.Context(0)
     ctx.set_options(pot_ghost_size=a.gs, gra_ghost_size=1, pot_scheme=pg.SOR, init_mode=pg.INIT_ZERO,
                     term_mode=pg.TERM_RELRES, check_every=a.check_every, rel_tol=1e-5, fuse_gravity=1)
+    if a.pad is not None or a.ppw is not None:
+        ppw0, pad0 = ctx.get_layout(a.gs)
+        ctx.set_layout(a.gs, a.ppw if a.ppw is not None else ppw0, a.pad if a.pad is not None else pad0)
     stream = torch.cuda.current_stream(dev)
 
     def step():

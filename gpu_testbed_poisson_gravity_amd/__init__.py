@@ -81,6 +81,8 @@ SIGNATURES = {
                                                     C.c_int, C.c_int, C.c_int]),
     "pgamd_synchronize": (C.c_int, [_CTX]),
     "pgamd_get_patch_stats": (C.c_int, [_CTX, _IP, _IP, C.c_int]),
+    "pgamd_set_layout": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_int]),
+    "pgamd_get_layout": (C.c_int, [_CTX, C.c_int, _IP, _IP]),
     "pgamd_solve_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                      C.POINTER(SolverArgs), C.c_void_p, C.c_void_p, C.c_void_p]),
 }
@@ -208,6 +210,14 @@ class Context:
         status = np.zeros(npatch, np.int32)
         _check(load().pgamd_get_patch_stats(self.h, iters.ctypes.data_as(_IP), status.ctypes.data_as(_IP), npatch))
         return iters, status
+
+    def set_layout(self, gs: int, ppw: int, pad: int):
+        _check(load().pgamd_set_layout(self.h, gs, ppw, pad))
+
+    def get_layout(self, gs: int):
+        a, b = C.c_int(), C.c_int()
+        _check(load().pgamd_get_layout(self.h, gs, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     # --- device-resident ----------------------------------------------------------------------------------
     def solve_device(self, rho, cpot, pot_out, flu, npatch, args: SolverArgs, guess=None, iters=None, status=None,

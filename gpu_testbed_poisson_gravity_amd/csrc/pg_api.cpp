@@ -189,8 +189,12 @@ int pgamd_set_diagnose_device(int device_id, int* npatch_group, int* nstream, pg
     return hip_fail(e, "hipGetDeviceProperties");
   }
   pgamd_default_options(&c->opt);
-  for (int g = 1; g <= 5; g++) c->layout.ppw[g] = default_ppw(g);
+  for (int g = 1; g <= 5; g++) {
+    c->layout.ppw[g] = default_ppw(g);
+    c->layout.pad[g] = default_pad(g);
+  }
   c->layout.ppw[0] = 1;
+  c->layout.pad[0] = 0;
   c->layout.waves_per_cu = 0;
   if (npatch_group && *npatch_group <= 0) *npatch_group = 32 * c->prop.multiProcessorCount;
   if (nstream && *nstream <= 0) *nstream = 1;
@@ -362,6 +366,24 @@ int pgamd_get_patch_stats(pgamd_context* c, int* iters, int* status, int npatch)
   if (npatch > c->last_npatch) return fail(PGAMD_ERR_ARG, "only %d patches in the last solve", c->last_npatch);
   if (iters) memcpy(iters, c->h_iters, sizeof(int) * npatch);
   if (status) memcpy(status, c->h_status, sizeof(int) * npatch);
+  return PGAMD_OK;
+}
+
+int pgamd_set_layout(pgamd_context* c, int gs, int ppw, int pad) {
+  if (!c) return fail(PGAMD_ERR_ARG, "null context");
+  if (gs < 1 || gs > kMaxGhost) return fail(PGAMD_ERR_ARG, "ghost size %d outside [1,5]", gs);
+  if (!supported_ppw(gs, ppw)) return fail(PGAMD_ERR_ARG, "%d patches per wavefront do not fit ghost size %d", ppw, gs);
+  if (pad < 0 || pad > 3) return fail(PGAMD_ERR_ARG, "row padding %d outside [0,3]", pad);
+  c->layout.ppw[gs] = ppw;
+  c->layout.pad[gs] = pad;
+  return PGAMD_OK;
+}
+
+int pgamd_get_layout(pgamd_context* c, int gs, int* ppw, int* pad) {
+  if (!c) return fail(PGAMD_ERR_ARG, "null context");
+  if (gs < 1 || gs > kMaxGhost) return fail(PGAMD_ERR_ARG, "ghost size %d outside [1,5]", gs);
+  if (ppw) *ppw = c->layout.ppw[gs];
+  if (pad) *pad = c->layout.pad[gs];
   return PGAMD_OK;
 }
 

@@ -23,7 +23,7 @@ hipStream_t ctx_stream(const pgamd_context* c);
 // Frees whatever the optional modules (graphs, mixed batches, CNN) attached to a context.
 void release_extras(pgamd_context* c);
 
-hipError_t launch_sor_uniform_dispatch(const UniformBatch& b, const SolveParams& p, int ppw, hipStream_t s);
+hipError_t launch_sor_uniform_dispatch(const UniformBatch& b, const SolveParams& p, int ppw, int pad, hipStream_t s);
 hipError_t launch_mg_uniform(const UniformBatch& b, const SolveParams& p, hipStream_t s);
 
 }  // namespace pgamd

@@ -91,7 +91,8 @@ struct MixedBatch {
 // Per-ghost-size launch layout of the SOR kernel: patches per wavefront.  Chosen by the autotuner;
 // the default is the densest packing that fits one wavefront.
 struct SorLayout {
-  int ppw[6];              // index by gs
+  int ppw[6];              // patches per wavefront, indexed by gs
+  int pad[6];              // split-row padding in floats (bank-conflict avoidance), indexed by gs
   int waves_per_cu;        // persistent kernel occupancy target
 };
 
@@ -104,7 +105,8 @@ hipError_t launch_mixed(const MixedBatch& b, const SolveParams* per_gs, const So
 hipError_t launch_reset_counter(unsigned int* counter, hipStream_t stream);
 int sor_lanes_per_patch(int gs);
 int default_ppw(int gs);
-size_t lds_bytes_sor(int gs, int ppw);
+int default_pad(int gs);
+size_t lds_bytes_sor(int gs, int ppw, int pad);
 size_t lds_bytes_mg(int gs);
 int supported_ppw(int gs, int ppw);
 

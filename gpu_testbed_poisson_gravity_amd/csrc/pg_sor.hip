@@ -36,18 +36,24 @@
 
 namespace pgamd {
 
-template <int GS>
+// Compile-time layout: L = gs + 8*pad selects the ghost size and the padding of the split rows
+// (HP = N/2 + pad floats per row).  Padding trades LDS capacity for fewer bank conflicts.
+template <int L>
 struct SorGeo {
-  static constexpr Geo g = geo_of(GS);
+  static constexpr int gs = L & 7;
+  static constexpr int PAD = L >> 3;
+  static constexpr Geo g = geo_of(gs);
   static constexpr int N = g.N;
   static constexpr int R = g.R;
   static constexpr int C = g.C;
   static constexpr int U = g.U;
-  static constexpr int HP = N / 2;              // split row length in floats
+  static constexpr int HP = N / 2 + PAD;        // split row stride in floats
   static constexpr int MX = R / 2;              // x-pairs per row
   static constexpr int LP = MX * (R / 2);       // lanes per patch
-  static constexpr int FLOATS = 2 * N * N * HP; // == N^3
+  static constexpr int FLOATS = 2 * N * N * HP; // LDS floats per patch
 };
+
+constexpr int layout_code(int gs, int pad) { return gs + 8 * pad; }
 
 __device__ __forceinline__ int cube_index(int N, int HP, int x, int y, int z) {
   return ((((x + y + z) & 1) * N + z) * N + y) * HP + (x >> 1);
@@ -61,6 +67,9 @@ __device__ __forceinline__ float seg_sum(float v) {
   for (int off = SEG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+// Any lane of the wavefront (a workgroup is exactly one wavefront here).
+__device__ __forceinline__ bool wave_any(bool pred) { return __ballot(pred) != 0; }
 
 // ---- interpolation into LDS --------------------------------------------------------------------------
 
@@ -111,50 +120,101 @@ __device__ __forceinline__ constexpr int own_off(int c, int z, int yy, int xx) {
   return ((c * S::N + z) * S::N + (yy + 1)) * S::HP + xx;
 }
 
-// Pre-update residuals of the lane's cells of colour COL (cube parity), reference operand order
+// Software pipeline depth of the LDS loads in a half-sweep: the two neighbour values of cell q+kLookahead
+// are requested before cell q is computed, so a wavefront keeps ~2*kLookahead LDS reads in flight while
+// holding only that many extra registers.
+constexpr int kLookahead = 8;
+
+// Cells of colour COL owned by a lane, enumerated z-major: q = 2*zi + yy.
+template <int GS, int COL>
+struct HalfSweep {
+  static constexpr int R = SorGeo<GS>::R;
+  static constexpr int NQ = 2 * R;
+  static constexpr int yy(int q) { return q & 1; }
+  static constexpr int zi(int q) { return q >> 1; }
+  static constexpr int xx(int q) { return (COL + (q & 1) + (q >> 1) + 1) & 1; }
+};
+
+// The two LDS operands of owned cell q: the neighbour across the y-pair and the one across the x-pair,
+// both in the other colour's array at row offset m+xx.
+template <int GS, int COL>
+__device__ __forceinline__ void load_nb(const float* __restrict__ lds, int lb, int q, float (&v)[2]) {
+  using H = HalfSweep<GS, COL>;
+  const int yy = H::yy(q), zi = H::zi(q), xx = H::xx(q);
+  v[0] = lds[lb + own_off<GS>(1 - COL, zi + 1, yy == 0 ? -1 : 2, xx)];
+  v[1] = lds[lb + own_off<GS>(1 - COL, zi + 1, yy, xx)];
+}
+
+// Pre-update residual of owned cell q of colour COL, reference operand order
 // (CPU_PoissonSolver_SOR.cpp:247-250): ((((((kp+km)+jp)+jm)+ip)+im) - 6 phi) - Const*rho.
 template <int GS, int COL>
-__device__ __forceinline__ void residuals(const LaneCells<GS>& L, const float* __restrict__ lds, int lb,
-                                          float (&res)[2][SorGeo<GS>::R]) {
-  constexpr int R = SorGeo<GS>::R;
+__device__ __forceinline__ float residual_at(const LaneCells<GS>& L, const float (&v)[2], int q) {
+  using H = HalfSweep<GS, COL>;
+  constexpr int R = H::R;
+  const int yy = H::yy(q), zi = H::zi(q), xx = H::xx(q);
+  const float kp = zi + 1 < R ? L.phi[yy][xx][zi + 1] : L.bz[yy][xx][1];
+  const float km = zi > 0 ? L.phi[yy][xx][zi - 1] : L.bz[yy][xx][0];
+  const float jp = yy == 0 ? L.phi[1][xx][zi] : v[0];
+  const float jm = yy == 1 ? L.phi[0][xx][zi] : v[0];
+  const float ip = xx == 0 ? L.phi[yy][1][zi] : v[1];
+  const float im = xx == 1 ? L.phi[yy][0][zi] : v[1];
+  return kp + km + jp + jm + ip + im - 6.0f * L.phi[yy][xx][zi] - L.cr[yy][xx][zi];
+}
+
+// Publish the lane's cells of colour COL to LDS (one exec-masked block per half-sweep).
+template <int GS, int COL>
+__device__ __forceinline__ void store_colour(const LaneCells<GS>& L, float* __restrict__ lds, int lb, bool active) {
+  using H = HalfSweep<GS, COL>;
+  if (active) {
 #pragma unroll
-  for (int yy = 0; yy < 2; yy++)
+    for (int q = 0; q < H::NQ; q++)
+      lds[lb + own_off<GS>(COL, H::zi(q) + 1, H::yy(q), H::xx(q))] = L.phi[H::yy(q)][H::xx(q)][H::zi(q)];
+  }
+}
+
+// Residuals of all owned cells of colour COL with pipelined loads.  MODE 0: phi += w*R in place (w is
+// omega/6 for running patches and 0 for converged ones: phi + 0*R == phi for the finite residuals of a
+// converged patch, so no branch is needed).  MODE 1: keep R in res[] and leave phi unchanged.
+// Returns sum|R| when ACC.
+template <int GS, int COL, int MODE, bool ACC>
+__device__ __forceinline__ float half_sweep(LaneCells<GS>& L, const float* __restrict__ lds, int lb, float w,
+                                            float (&res)[2 * SorGeo<GS>::R]) {
+  using H = HalfSweep<GS, COL>;
+  constexpr int NQ = H::NQ;
+  float nb[NQ][2];
 #pragma unroll
-    for (int zi = 0; zi < R; zi++) {
-      const int xx = (COL + yy + zi + 1) & 1;
-      const int z = zi + 1;
-      const float kp = zi + 1 < R ? L.phi[yy][xx][zi + 1] : L.bz[yy][xx][1];
-      const float km = zi > 0 ? L.phi[yy][xx][zi - 1] : L.bz[yy][xx][0];
-      const float jp = yy == 0 ? L.phi[1][xx][zi] : lds[lb + own_off<GS>(1 - COL, z, 2, xx)];
-      const float jm = yy == 1 ? L.phi[0][xx][zi] : lds[lb + own_off<GS>(1 - COL, z, -1, xx)];
-      const float ip = xx == 0 ? L.phi[yy][1][zi] : lds[lb + own_off<GS>(1 - COL, z, yy, xx)];
-      const float im = xx == 1 ? L.phi[yy][0][zi] : lds[lb + own_off<GS>(1 - COL, z, yy, xx)];
-      res[yy][zi] = kp + km + jp + jm + ip + im - 6.0f * L.phi[yy][xx][zi] - L.cr[yy][xx][zi];
+  for (int q = 0; q < kLookahead && q < NQ; q++) load_nb<GS, COL>(lds, lb, q, nb[q]);
+  float acc = 0.0f;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    if (q + kLookahead < NQ) load_nb<GS, COL>(lds, lb, q + kLookahead, nb[q + kLookahead]);
+    const float r = residual_at<GS, COL>(L, nb[q], q);
+    if (ACC) acc += fabsf(r);
+    if (MODE == 0) {
+      float& ph = L.phi[H::yy(q)][H::xx(q)][H::zi(q)];
+      ph = ph + w * r;
+    } else {
+      res[q] = r;
     }
+  }
+  return acc;
 }
 
 template <int GS, int COL>
-__device__ __forceinline__ void apply(LaneCells<GS>& L, float* __restrict__ lds, int lb,
-                                      const float (&res)[2][SorGeo<GS>::R], float w6) {
-  constexpr int R = SorGeo<GS>::R;
+__device__ __forceinline__ void apply_res(LaneCells<GS>& L, const float (&res)[2 * SorGeo<GS>::R], float w) {
+  using H = HalfSweep<GS, COL>;
 #pragma unroll
-  for (int yy = 0; yy < 2; yy++)
-#pragma unroll
-    for (int zi = 0; zi < R; zi++) {
-      const int xx = (COL + yy + zi + 1) & 1;
-      const float v = L.phi[yy][xx][zi] + w6 * res[yy][zi];
-      L.phi[yy][xx][zi] = v;
-      lds[lb + own_off<GS>(COL, zi + 1, yy, xx)] = v;
-    }
+  for (int q = 0; q < H::NQ; q++) {
+    float& ph = L.phi[H::yy(q)][H::xx(q)][H::zi(q)];
+    ph = ph + w * res[q];
+  }
 }
 
 template <int GS>
-__device__ __forceinline__ float abs_sum(const float (&res)[2][SorGeo<GS>::R]) {
+__device__ __forceinline__ float abs_sum(const float (&res)[2 * SorGeo<GS>::R]) {
   float s = 0.0f;
 #pragma unroll
-  for (int yy = 0; yy < 2; yy++)
-#pragma unroll
-    for (int zi = 0; zi < SorGeo<GS>::R; zi++) s += fabsf(res[yy][zi]);
+  for (int q = 0; q < 2 * SorGeo<GS>::R; q++) s += fabsf(res[q]);
   return s;
 }
 
@@ -178,7 +238,7 @@ __device__ void gravity_from_lds(const float* __restrict__ lds, float* __restric
   constexpr int N = S::N, HP = S::HP;
   for (int t = lane; t < kPatchCells; t += nlanes) {
     const int i = t % kPatchSize, j = (t / kPatchSize) % kPatchSize, k = t / (kPatchSize * kPatchSize);
-    const int x = i + GS, y = j + GS, z = k + GS;
+    const int x = i + S::gs, y = j + S::gs, z = k + S::gs;
     auto P = [&](int dz, int dy, int dx) { return lds[cube_index(N, HP, x + dx, y + dy, z + dz)]; };
     float ax, ay, az;
     if (p5) {
@@ -255,12 +315,13 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
   const bool relres = P.term_mode == 1, stall = P.term_mode == 0;
   const int first_check = P.min_iter > 1 ? P.min_iter : 1;
   float sum_b = 0.0f, old = FLT_MAX;
-  float res[2][R];
+  float res[2 * R];
   for (int t = 0; t < P.max_iter; ++t) {
-    residuals<GS, 1>(L, lds, lb, res);
+    const bool check = relres && t >= first_check && (t - first_check) % P.check_every == 0;
+    const bool pre_check = relres && t + 1 >= first_check && (t + 1 - first_check) % P.check_every == 0;
     float tot = 0.0f;
-    if (relres && t >= first_check && (t - first_check) % P.check_every == 0) {
-      float num = abs_sum<GS>(res), den = 0.0f;
+    if (check) {
+      float num = half_sweep<GS, 1, 1, true>(L, lds, lb, 0.0f, res), den = 0.0f;
 #pragma unroll
       for (int yy = 0; yy < 2; yy++)
 #pragma unroll
@@ -276,16 +337,24 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
         it_done = t;
         st = kStatusConverged;
       }
-      if (!__syncthreads_or(running)) break;
+      if (!wave_any(running)) break;
+      apply_res<GS, 1>(L, res, running ? P.w6 : 0.0f);
     } else if (stall) {
-      tot = abs_sum<GS>(res);
+      tot = half_sweep<GS, 1, 0, true>(L, lds, lb, running ? P.w6 : 0.0f, res);
+    } else {
+      half_sweep<GS, 1, 0, false>(L, lds, lb, running ? P.w6 : 0.0f, res);
     }
-    if (running) apply<GS, 1>(L, lds, lb, res, P.w6);
+    store_colour<GS, 1>(L, lds, lb, active);
     __syncthreads();
-    residuals<GS, 0>(L, lds, lb, res);
-    if (relres && t + 1 >= first_check && (t + 1 - first_check) % P.check_every == 0) sum_b = abs_sum<GS>(res);
-    if (stall) tot += abs_sum<GS>(res);
-    if (running) apply<GS, 0>(L, lds, lb, res, P.w6);
+    const float w = running ? P.w6 : 0.0f;
+    if (stall || pre_check) {
+      const float s2 = half_sweep<GS, 0, 0, true>(L, lds, lb, w, res);
+      sum_b = s2;
+      tot += s2;
+    } else {
+      half_sweep<GS, 0, 0, false>(L, lds, lb, w, res);
+    }
+    store_colour<GS, 0>(L, lds, lb, active);
     __syncthreads();
     if (stall) {
       tot = seg_sum<SEG>(active ? tot : 0.0f);
@@ -295,7 +364,7 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
         st = kStatusConverged;
       }
       old = tot;
-      if (!__syncthreads_or(running)) break;
+      if (!wave_any(running)) break;
     }
   }
 
@@ -318,7 +387,7 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
 
   // D. output window: out[k][j][i] = cube[k+o][j+o][i+o], o = GS - GRA_GHOST_SIZE
   if (lp.valid) {
-    const int G = gra_nxt(P.gra_ghost), o = GS - P.gra_ghost;
+    const int G = gra_nxt(P.gra_ghost), o = S::gs - P.gra_ghost;
     for (int t = pl; t < G * G * G; t += SEG) {
       const int i = t % G, j = (t / G) % G, k = t / (G * G);
       lp.out[t] = lds[cube_index(N, HP, i + o, j + o, k + o)];
@@ -348,7 +417,7 @@ __device__ __forceinline__ LanePatch uniform_patch(const UniformBatch& b, int pa
 }
 
 template <int GS, int PPW>
-__global__ void __launch_bounds__(64) sor_uniform_kernel(UniformBatch b, SolveParams P) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) sor_uniform_kernel(UniformBatch b, SolveParams P) {
   extern __shared__ float lds[];
   constexpr int SEG = 64 / PPW;
   const int slot = threadIdx.x / SEG;
@@ -414,29 +483,46 @@ int default_ppw(int gs) {
   return 1;
 }
 
-size_t lds_bytes_sor(int gs, int ppw) {
+// Measured on MI355X (16384 patches, 1e-5): ghost size 4 gains 17% with 2 floats of row padding (HP = 10
+// breaks the 16-float row stride that puts every other lane row on the same banks); the other ghost sizes
+// are best unpadded, where padding costs a resident patch per CU.
+int default_pad(int gs) { return gs == 4 ? 2 : 0; }
+
+size_t lds_bytes_sor(int gs, int ppw, int pad) {
   const Geo g = geo_of(gs);
-  return sizeof(float) * (size_t)ppw * g.N * g.N * g.N;
+  return sizeof(float) * (size_t)ppw * 2 * g.N * g.N * (g.N / 2 + pad);
 }
 
-template <int GS, int PPW>
+template <int GS, int PPW, int PAD>
 static hipError_t launch_sor_uniform(const UniformBatch& b, const SolveParams& p, hipStream_t s) {
+  constexpr int L = layout_code(GS, PAD);
   const int blocks = (b.npatch + PPW - 1) / PPW;
-  const size_t lds = lds_bytes_sor(GS, PPW);
-  hipLaunchKernelGGL((sor_uniform_kernel<GS, PPW>), dim3(blocks), dim3(64), lds, s, b, p);
+  const size_t lds = lds_bytes_sor(GS, PPW, PAD);
+  hipLaunchKernelGGL((sor_uniform_kernel<L, PPW>), dim3(blocks), dim3(64), lds, s, b, p);
   return hipGetLastError();
 }
 
-hipError_t launch_sor_uniform_dispatch(const UniformBatch& b, const SolveParams& p, int ppw, hipStream_t s) {
+template <int GS, int PPW>
+static hipError_t launch_sor_pad(const UniformBatch& b, const SolveParams& p, int pad, hipStream_t s) {
+  switch (pad) {
+    case 0: return launch_sor_uniform<GS, PPW, 0>(b, p, s);
+    case 1: return launch_sor_uniform<GS, PPW, 1>(b, p, s);
+    case 2: return launch_sor_uniform<GS, PPW, 2>(b, p, s);
+    case 3: return launch_sor_uniform<GS, PPW, 3>(b, p, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_sor_uniform_dispatch(const UniformBatch& b, const SolveParams& p, int ppw, int pad, hipStream_t s) {
   switch (b.gs * 8 + ppw) {
-    case 1 * 8 + 1: return launch_sor_uniform<1, 1>(b, p, s);
-    case 1 * 8 + 2: return launch_sor_uniform<1, 2>(b, p, s);
-    case 1 * 8 + 4: return launch_sor_uniform<1, 4>(b, p, s);
-    case 2 * 8 + 1: return launch_sor_uniform<2, 1>(b, p, s);
-    case 2 * 8 + 2: return launch_sor_uniform<2, 2>(b, p, s);
-    case 3 * 8 + 1: return launch_sor_uniform<3, 1>(b, p, s);
-    case 4 * 8 + 1: return launch_sor_uniform<4, 1>(b, p, s);
-    case 5 * 8 + 1: return launch_sor_uniform<5, 1>(b, p, s);
+    case 1 * 8 + 1: return launch_sor_pad<1, 1>(b, p, pad, s);
+    case 1 * 8 + 2: return launch_sor_pad<1, 2>(b, p, pad, s);
+    case 1 * 8 + 4: return launch_sor_pad<1, 4>(b, p, pad, s);
+    case 2 * 8 + 1: return launch_sor_pad<2, 1>(b, p, pad, s);
+    case 2 * 8 + 2: return launch_sor_pad<2, 2>(b, p, pad, s);
+    case 3 * 8 + 1: return launch_sor_pad<3, 1>(b, p, pad, s);
+    case 4 * 8 + 1: return launch_sor_pad<4, 1>(b, p, pad, s);
+    case 5 * 8 + 1: return launch_sor_pad<5, 1>(b, p, pad, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -444,7 +530,7 @@ hipError_t launch_sor_uniform_dispatch(const UniformBatch& b, const SolveParams&
 hipError_t launch_uniform(const UniformBatch& b, const SolveParams& p, const SorLayout& lay, hipStream_t s) {
   if (b.npatch <= 0) return hipSuccess;
   if (b.scheme == kSchemeMG) return launch_mg_uniform(b, p, s);
-  return launch_sor_uniform_dispatch(b, p, lay.ppw[b.gs], s);
+  return launch_sor_uniform_dispatch(b, p, lay.ppw[b.gs], lay.pad[b.gs], s);
 }
 
 hipError_t launch_gravity(const float* pot, float* flu, int npatch, int gra_ghost, float gra_const, int p5,

@@ -109,6 +109,13 @@ int pgamd_synchronize(pgamd_context* ctx);
  * (CPU_PoissonSolver_SOR.cpp:290). */
 int pgamd_get_patch_stats(pgamd_context* ctx, int* iters, int* status, int npatch);
 
+/* Launch layout of the SOR kernel for one ghost size: patches per wavefront (1, 2 or 4 where they fit) and
+ * the padding of the colour-split LDS rows (0..3 floats; trades LDS capacity for fewer bank conflicts).
+ * This is the autotuner's search space (pgamd_autotune); no reference counterpart: the reference fixes its
+ * launch shapes per GPU generation with macros (Header/CUPOT.h:38-152). */
+int pgamd_set_layout(pgamd_context* ctx, int ghost_size, int patches_per_wave, int row_pad);
+int pgamd_get_layout(pgamd_context* ctx, int ghost_size, int* patches_per_wave, int* row_pad);
+
 /* ---- device-resident entry points (inputs already in HBM; no reference counterpart: the reference
  * always stages through host memory).  All pointers are device pointers; `stream` is a hipStream_t
  * (NULL = the HIP default stream).  iters / status may be NULL. */
