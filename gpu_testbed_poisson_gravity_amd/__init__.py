@@ -85,6 +85,21 @@ SIGNATURES = {
     "pgamd_get_layout": (C.c_int, [_CTX, C.c_int, _IP, _IP]),
     "pgamd_solve_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                      C.POINTER(SolverArgs), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pgamd_plan_create": (C.c_int, [_CTX, _IP, _IP, C.c_int, C.POINTER(C.c_void_p)]),
+    "pgamd_plan_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "pgamd_plan_offsets": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "pgamd_plan_destroy": (C.c_int, [C.c_void_p]),
+    "pgamd_solve_mixed_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.POINTER(SolverArgs), _FP, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
+    "pgamd_graph_capture_mixed": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.POINTER(SolverArgs), _FP, C.c_void_p, C.c_void_p,
+                                            C.POINTER(C.c_void_p)]),
+    "pgamd_graph_capture_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_int, C.POINTER(SolverArgs), C.c_void_p, C.c_void_p,
+                                             C.POINTER(C.c_void_p)]),
+    "pgamd_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pgamd_graph_destroy": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None
@@ -97,6 +112,13 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is missing: build it with `make -C {HERE}` (no CPU fallback exists)")
+    # PyTorch wheels bundle their own HIP runtime under the same soname.  Whichever copy is loaded first
+    # serves the whole process; torch only initialises on its own copy, so load torch (when present)
+    # before the library resolves libamdhip64.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
@@ -224,6 +246,77 @@ class Context:
                      stream=None):
         _check(load().pgamd_solve_device(self.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out), _vp(flu), npatch,
                                          C.byref(args), _vp(iters), _vp(status), stream))
+
+    # --- mixed batches and graphs ---------------------------------------------------------------------------
+    def plan(self, ghost_sizes, schemes=None) -> "Plan":
+        return Plan(self, ghost_sizes, schemes)
+
+    def solve_mixed_device(self, plan, rho, cpot, pot_out, flu, args: SolverArgs, guess=None, omega_by_ghost=None,
+                           iters=None, status=None, stream=None):
+        om = None if omega_by_ghost is None else (C.c_float * 5)(*omega_by_ghost)
+        _check(load().pgamd_solve_mixed_device(self.h, plan.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out),
+                                               _vp(flu), C.byref(args), om, _vp(iters), _vp(status), stream))
+
+    def graph_mixed(self, plan, rho, cpot, pot_out, flu, args: SolverArgs, guess=None, omega_by_ghost=None,
+                    iters=None, status=None) -> "Graph":
+        om = None if omega_by_ghost is None else (C.c_float * 5)(*omega_by_ghost)
+        h = C.c_void_p()
+        _check(load().pgamd_graph_capture_mixed(self.h, plan.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out),
+                                                _vp(flu), C.byref(args), om, _vp(iters), _vp(status), C.byref(h)))
+        return Graph(h)
+
+    def graph_device(self, rho, cpot, pot_out, flu, npatch, args: SolverArgs, guess=None, iters=None,
+                     status=None) -> "Graph":
+        h = C.c_void_p()
+        _check(load().pgamd_graph_capture_device(self.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out), _vp(flu),
+                                                 npatch, C.byref(args), _vp(iters), _vp(status), C.byref(h)))
+        return Graph(h)
+
+
+class Plan:
+    """pgamd_plan_*: per-patch ghost sizes and schemes, packed offsets and the persistent work queues."""
+
+    def __init__(self, ctx: Context, ghost_sizes, schemes=None):
+        gs = np.ascontiguousarray(ghost_sizes, dtype=np.int32)
+        sc = None if schemes is None else np.ascontiguousarray(schemes, dtype=np.int32)
+        self.npatch = int(gs.size)
+        self.h = C.c_void_p()
+        _check(load().pgamd_plan_create(ctx.h, gs.ctypes.data_as(_IP), None if sc is None else sc.ctypes.data_as(_IP),
+                                        self.npatch, C.byref(self.h)))
+        r, c = C.c_int64(), C.c_int64()
+        _check(load().pgamd_plan_sizes(self.h, C.byref(r), C.byref(c)))
+        self.rho_floats, self.pot_in_floats = r.value, c.value
+        self.rho_off = np.zeros(self.npatch, np.int64)
+        self.pot_in_off = np.zeros(self.npatch, np.int64)
+        _check(load().pgamd_plan_offsets(self.h, self.rho_off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         self.pot_in_off.ctypes.data_as(C.POINTER(C.c_int64))))
+
+    def close(self):
+        if self.h:
+            _check(load().pgamd_plan_destroy(self.h))
+            self.h = None
+
+
+class Graph:
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self, stream=None):
+        _check(load().pgamd_graph_launch(self.h, stream))
+
+    def close(self):
+        if self.h:
+            _check(load().pgamd_graph_destroy(self.h))
+            self.h = None
+
+
+def pack(arrays, offsets, total):
+    """Pack per-patch arrays (any shapes) into one float32 vector at the given element offsets."""
+    out = np.zeros(total, np.float32)
+    for a, o in zip(arrays, offsets):
+        f = np.asarray(a, np.float32).ravel()
+        out[o:o + f.size] = f
+    return out
 
 
 def solve_host(rho, cpot, flu, *, gs, gg=1, scheme=SOR, init_mode=INIT_INTERP, term_mode=TERM_REFERENCE,

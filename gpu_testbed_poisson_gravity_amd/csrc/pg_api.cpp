@@ -127,6 +127,9 @@ struct pgamd_context {
   int* h_status = nullptr;
   int last_npatch = 0;
   size_t rho_n = 0, cpot_n = 0, out_n = 0;  // floats per patch at allocation time
+  hipStream_t aux = nullptr;                // forked launches (mixed batches)
+  hipStream_t capture = nullptr;            // graph capture
+  hipEvent_t ev[2] = {nullptr, nullptr};    // fork / join
 };
 
 namespace {
@@ -214,6 +217,10 @@ int pgamd_destroy(pgamd_context* c) {
   free_device(c);
   pgamd::release_extras(c);
   for (auto s : c->streams) hipStreamDestroy(s);
+  if (c->aux) hipStreamDestroy(c->aux);
+  if (c->capture) hipStreamDestroy(c->capture);
+  for (auto e : c->ev)
+    if (e) hipEventDestroy(e);
   delete c;
   return PGAMD_OK;
 }
@@ -414,6 +421,21 @@ SorLayout& ctx_layout(pgamd_context* c) { return c->layout; }
 int ctx_cu_count(const pgamd_context* c) { return c->prop.multiProcessorCount; }
 int ctx_device(const pgamd_context* c) { return c->device; }
 hipStream_t ctx_stream(const pgamd_context* c) { return c->streams[0]; }
+
+hipStream_t ctx_aux_stream(pgamd_context* c) {
+  if (!c->aux && hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess) c->aux = nullptr;
+  return c->aux;
+}
+
+hipStream_t ctx_capture_stream(pgamd_context* c) {
+  if (!c->capture && hipStreamCreateWithFlags(&c->capture, hipStreamNonBlocking) != hipSuccess) c->capture = nullptr;
+  return c->capture;
+}
+
+hipEvent_t ctx_event(pgamd_context* c, int i) {
+  if (!c->ev[i] && hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming) != hipSuccess) c->ev[i] = nullptr;
+  return c->ev[i];
+}
 
 // Poisson (SOR or MG) then, unless fused, the standalone gravity kernel: the order of
 // CUAPI_Asyn_PoissonGravitySolver.cu:210-246.

@@ -20,6 +20,11 @@ SorLayout& ctx_layout(pgamd_context* c);
 int ctx_cu_count(const pgamd_context* c);
 int ctx_device(const pgamd_context* c);
 hipStream_t ctx_stream(const pgamd_context* c);
+// Created on first use and owned by the context: a second stream for forked launches, a stream used
+// only for graph capture, and two events for fork/join.
+hipStream_t ctx_aux_stream(pgamd_context* c);
+hipStream_t ctx_capture_stream(pgamd_context* c);
+hipEvent_t ctx_event(pgamd_context* c, int i);
 // Frees whatever the optional modules (graphs, mixed batches, CNN) attached to a context.
 void release_extras(pgamd_context* c);
 

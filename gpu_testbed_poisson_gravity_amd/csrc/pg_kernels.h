@@ -88,6 +88,11 @@ struct MixedBatch {
   unsigned int* counter;   // work-queue head, reset by the launcher
 };
 
+// Solver settings per ghost size for a mixed batch (omega may differ per ghost size).
+struct ParamsByGs {
+  SolveParams p[6];
+};
+
 // Per-ghost-size launch layout of the SOR kernel: patches per wavefront.  Chosen by the autotuner;
 // the default is the densest packing that fits one wavefront.
 struct SorLayout {
@@ -100,9 +105,15 @@ struct SorLayout {
 hipError_t launch_uniform(const UniformBatch& b, const SolveParams& p, const SorLayout& lay, hipStream_t stream);
 hipError_t launch_gravity(const float* pot, float* flu, int npatch, int gra_ghost, float gra_const, int p5,
                           hipStream_t stream);
-hipError_t launch_mixed(const MixedBatch& b, const SolveParams* per_gs, const SorLayout& lay, int grid,
-                        hipStream_t stream);
-hipError_t launch_reset_counter(unsigned int* counter, hipStream_t stream);
+// Persistent kernels of a mixed batch: SOR tasks (one wavefront each, gs 1/2 packed ppw1/ppw2 per
+// wavefront, default row padding) and MG tasks (one 256-thread workgroup per patch).
+hipError_t launch_sor_persistent(const MixedBatch& b, const ParamsByGs& p, int ppw1, int ppw2, int grid, size_t lds,
+                                 hipStream_t stream);
+hipError_t launch_mg_persistent(const MixedBatch& b, const ParamsByGs& p, int grid, size_t lds, hipStream_t stream);
+int sor_persistent_blocks_per_cu(int ppw1, int ppw2, size_t lds);
+int mg_persistent_blocks_per_cu(size_t lds);
+// Zeroes n work-queue counters with a kernel (graph-capture safe, no memset node).
+hipError_t launch_reset_counters(unsigned int* counters, int n, hipStream_t stream);
 int sor_lanes_per_patch(int gs);
 int default_ppw(int gs);
 int default_pad(int gs);

@@ -422,6 +422,53 @@ static hipError_t launch_mg(const UniformBatch& b, const SolveParams& p, hipStre
   return hipGetLastError();
 }
 
+// Persistent MG kernel of a mixed batch: each workgroup pulls one patch at a time from the queue.
+__global__ void __launch_bounds__(kMgThreads) mg_persistent_kernel(MixedBatch b, ParamsByGs P) {
+  extern __shared__ float lds[];
+  __shared__ int s_task;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = (int)atomicAdd(b.counter, 1u);
+    __syncthreads();
+    const int t = s_task;
+    __syncthreads();
+    if (t >= b.ntasks) break;
+    const Task tk = b.tasks[t];
+    const int patch = b.perm[tk.first];
+    if (patch < 0 || patch >= b.npatch || tk.gs < 1 || tk.gs > kMaxGhost) continue;  // malformed plan: skip
+    const PatchDesc d = b.desc[patch];
+    const SolveParams& Q = P.p[tk.gs];
+    const int G = gra_nxt(Q.gra_ghost);
+    const float* rho = b.rho + d.rho_off;
+    const float* cpot = b.cpot + d.cpot_off;
+    const float* guess = b.guess ? b.guess + d.rho_off : nullptr;
+    float* out = b.pot_out + (size_t)patch * G * G * G;
+    float* flu = b.flu ? b.flu + (size_t)patch * kNComp * kPatchCells : nullptr;
+    int* it = b.iters ? b.iters + patch : nullptr;
+    int* st = b.status ? b.status + patch : nullptr;
+    switch (tk.gs) {
+      case 1: mg_patch<1>(lds, rho, cpot, guess, out, flu, it, st, Q); break;
+      case 2: mg_patch<2>(lds, rho, cpot, guess, out, flu, it, st, Q); break;
+      case 3: mg_patch<3>(lds, rho, cpot, guess, out, flu, it, st, Q); break;
+      case 4: mg_patch<4>(lds, rho, cpot, guess, out, flu, it, st, Q); break;
+      default: mg_patch<5>(lds, rho, cpot, guess, out, flu, it, st, Q); break;
+    }
+  }
+}
+
+int mg_persistent_blocks_per_cu(size_t lds) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&mg_persistent_kernel),
+                                                   kMgThreads, lds) != hipSuccess)
+    return 0;
+  return n;
+}
+
+hipError_t launch_mg_persistent(const MixedBatch& b, const ParamsByGs& p, int grid, size_t lds, hipStream_t s) {
+  if (b.ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(mg_persistent_kernel, dim3(grid), dim3(kMgThreads), lds, s, b, p);
+  return hipGetLastError();
+}
+
 hipError_t launch_mg_uniform(const UniformBatch& b, const SolveParams& p, hipStream_t s) {
   switch (b.gs) {
     case 1: return launch_mg<1>(b, p, s);

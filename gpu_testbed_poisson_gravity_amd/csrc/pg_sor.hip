@@ -465,6 +465,90 @@ __global__ void __launch_bounds__(256) gravity_kernel(const float* __restrict__ 
   }
 }
 
+// ---- persistent kernel over a mixed batch ----------------------------------------------------------------
+
+template <int L, int PPW>
+__device__ __forceinline__ void mixed_sor_task(float* __restrict__ lds, const MixedBatch& b, const Task& tk,
+                                               const SolveParams& P) {
+  constexpr int SEG = 64 / PPW;
+  const int slot = threadIdx.x / SEG;
+  LanePatch lp;
+  int patch = b.perm[tk.first + (slot < tk.count ? slot : 0)];
+  lp.valid = slot < tk.count && patch >= 0 && patch < b.npatch;  // a malformed plan never writes out of range
+  if (!lp.valid) patch = 0;
+  const PatchDesc d = b.desc[patch];
+  const int G = gra_nxt(P.gra_ghost);
+  lp.rho = b.rho + d.rho_off;
+  lp.cpot = b.cpot + d.cpot_off;
+  lp.guess = b.guess ? b.guess + d.rho_off : nullptr;
+  lp.out = b.pot_out + (size_t)patch * G * G * G;
+  lp.flu = b.flu ? b.flu + (size_t)patch * kNComp * kPatchCells : nullptr;
+  lp.iters = b.iters ? b.iters + patch : nullptr;
+  lp.status = b.status ? b.status + patch : nullptr;
+  sor_task<L, PPW>(lds, lp, P);
+}
+
+// Every wavefront-sized workgroup pulls tasks from one queue until it is empty (tasks are ordered largest
+// first on the host, so the tail of the queue is cheap).  LDS is sized for the largest ghost size present.
+template <int PPW1, int PPW2>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    sor_persistent_kernel(MixedBatch b, ParamsByGs P) {
+  extern __shared__ float lds[];
+  for (;;) {
+    int t = 0;
+    if (threadIdx.x == 0) t = (int)atomicAdd(b.counter, 1u);
+    t = __shfl(t, 0, 64);
+    if (t >= b.ntasks) break;
+    const Task tk = b.tasks[t];
+    switch (tk.gs) {
+      case 1: mixed_sor_task<layout_code(1, 0), PPW1>(lds, b, tk, P.p[1]); break;
+      case 2: mixed_sor_task<layout_code(2, 0), PPW2>(lds, b, tk, P.p[2]); break;
+      case 3: mixed_sor_task<layout_code(3, 0), 1>(lds, b, tk, P.p[3]); break;
+      case 4: mixed_sor_task<layout_code(4, 2), 1>(lds, b, tk, P.p[4]); break;
+      default: mixed_sor_task<layout_code(5, 0), 1>(lds, b, tk, P.p[5]); break;
+    }
+  }
+}
+
+__global__ void reset_counters_kernel(unsigned int* c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+hipError_t launch_reset_counters(unsigned int* counters, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n);
+  return hipGetLastError();
+}
+
+template <int PPW1, int PPW2>
+static const void* sor_persistent_fn() {
+  return reinterpret_cast<const void*>(&sor_persistent_kernel<PPW1, PPW2>);
+}
+
+static const void* sor_persistent_select(int ppw1, int ppw2) {
+  if (ppw1 == 4 && ppw2 == 2) return sor_persistent_fn<4, 2>();
+  if (ppw1 == 2 && ppw2 == 1) return sor_persistent_fn<2, 1>();
+  return nullptr;
+}
+
+int sor_persistent_blocks_per_cu(int ppw1, int ppw2, size_t lds) {
+  const void* fn = sor_persistent_select(ppw1, ppw2);
+  int n = 0;
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds) != hipSuccess) return 0;
+  return n;
+}
+
+hipError_t launch_sor_persistent(const MixedBatch& b, const ParamsByGs& p, int ppw1, int ppw2, int grid, size_t lds,
+                                 hipStream_t stream) {
+  if (b.ntasks <= 0) return hipSuccess;
+  if (ppw1 == 4 && ppw2 == 2)
+    hipLaunchKernelGGL((sor_persistent_kernel<4, 2>), dim3(grid), dim3(64), lds, stream, b, p);
+  else if (ppw1 == 2 && ppw2 == 1)
+    hipLaunchKernelGGL((sor_persistent_kernel<2, 1>), dim3(grid), dim3(64), lds, stream, b, p);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 // ---- host launchers ------------------------------------------------------------------------------------
 
 int sor_lanes_per_patch(int gs) {

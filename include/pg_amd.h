@@ -135,6 +135,34 @@ int pgamd_solve_device(pgamd_context* ctx, const float* d_rho, const float* d_po
                        float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* args, int* d_iters,
                        int* d_status, void* stream);
 
+/* ---- mixed batches (no reference counterpart: the reference compiles one ghost size and one scheme per
+ * binary).  A plan fixes the ghost size (1..5) and scheme (SOR/MG) of every patch, the packed layout of
+ * the density and coarse-potential arrays (patch p starts at rho_off[p] / pot_in_off[p] floats) and the
+ * work queue of the persistent kernels.  pot_out is [npatch][G^3] and flu [npatch][5][8^3] as usual.
+ * Context options other than pot_ghost_size / pot_scheme apply to every patch. */
+typedef struct pgamd_plan pgamd_plan;
+int pgamd_plan_create(pgamd_context* ctx, const int* ghost_size, const int* scheme, int npatch, pgamd_plan** plan);
+int pgamd_plan_sizes(const pgamd_plan* plan, int64_t* rho_floats, int64_t* pot_in_floats);
+int pgamd_plan_offsets(const pgamd_plan* plan, int64_t* rho_off, int64_t* pot_in_off);
+int pgamd_plan_destroy(pgamd_plan* plan);
+/* One persistent launch per scheme on forked streams.  omega_by_ghost[gs-1] (may be NULL) overrides
+ * args->sor_omega per ghost size. */
+int pgamd_solve_mixed_device(pgamd_context* ctx, pgamd_plan* plan, const float* d_rho, const float* d_pot_in,
+                             const float* d_guess, float* d_pot_out, float* d_flu, const pgamd_solver_args* args,
+                             const float* omega_by_ghost, int* d_iters, int* d_status, void* stream);
+
+/* ---- HIP graphs: capture a solve once, replay it with one launch (inputs are read from the same device
+ * buffers at every replay).  No reference counterpart. */
+typedef struct pgamd_graph pgamd_graph;
+int pgamd_graph_capture_mixed(pgamd_context* ctx, pgamd_plan* plan, const float* d_rho, const float* d_pot_in,
+                              const float* d_guess, float* d_pot_out, float* d_flu, const pgamd_solver_args* args,
+                              const float* omega_by_ghost, int* d_iters, int* d_status, pgamd_graph** graph);
+int pgamd_graph_capture_device(pgamd_context* ctx, const float* d_rho, const float* d_pot_in, const float* d_guess,
+                               float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* args,
+                               int* d_iters, int* d_status, pgamd_graph** graph);
+int pgamd_graph_launch(pgamd_graph* graph, void* stream);
+int pgamd_graph_destroy(pgamd_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif
