@@ -19,7 +19,7 @@ SOR, MG = 1, 2
 INT_CQUAD, INT_QUAD = 4, 5
 INIT_INTERP, INIT_ZERO, INIT_GUESS = 0, 1, 2
 TERM_REFERENCE, TERM_RELRES = 0, 1
-STATUS_CONVERGED, STATUS_MAX_ITER, STATUS_NONFINITE = 1, 2, 4
+STATUS_CONVERGED, STATUS_MAX_ITER, STATUS_NONFINITE, STATUS_FALLBACK = 1, 2, 4, 8
 ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_NOMEM = -1, -2, -3, -4
 
 
@@ -39,6 +39,7 @@ class Options(C.Structure):
         ("check_every", C.c_int),
         ("rel_tol", C.c_float),
         ("fuse_gravity", C.c_int),
+        ("guess_fallback_tol", C.c_float),
     ]
 
 
@@ -85,6 +86,10 @@ SIGNATURES = {
     "pgamd_get_layout": (C.c_int, [_CTX, C.c_int, _IP, _IP]),
     "pgamd_solve_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                      C.POINTER(SolverArgs), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pgamd_autotune": (C.c_int, [_CTX, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(SolverArgs),
+                                 C.c_int, _IP, _IP, _FP]),
+    "pgamd_autotune_save": (C.c_int, [_CTX, C.c_char_p]),
+    "pgamd_autotune_load": (C.c_int, [_CTX, C.c_char_p]),
     "pgamd_plan_create": (C.c_int, [_CTX, _IP, _IP, C.c_int, C.POINTER(C.c_void_p)]),
     "pgamd_plan_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "pgamd_plan_offsets": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
@@ -100,6 +105,20 @@ SIGNATURES = {
                                              C.POINTER(C.c_void_p)]),
     "pgamd_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pgamd_graph_destroy": (C.c_int, [C.c_void_p]),
+    "pgamd_cnn_num_params": (C.c_int, []),
+    "pgamd_cnn_create": (C.c_int, [_CTX, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pgamd_cnn_destroy": (C.c_int, [C.c_void_p]),
+    "pgamd_cnn_get_params": (C.c_int, [C.c_void_p, _FP, C.c_int]),
+    "pgamd_cnn_set_params": (C.c_int, [C.c_void_p, _FP, C.c_int]),
+    "pgamd_cnn_prepare": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(SolverArgs),
+                                    C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pgamd_cnn_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "pgamd_cnn_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _FP, C.c_void_p,
+                                      C.c_void_p]),
+    "pgamd_cnn_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, _FP,
+                                       C.c_void_p]),
+    "pgamd_cnn_warm_start": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(SolverArgs),
+                                       C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -247,6 +266,19 @@ class Context:
         _check(load().pgamd_solve_device(self.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out), _vp(flu), npatch,
                                          C.byref(args), _vp(iters), _vp(status), stream))
 
+    def autotune(self, gs, rho, cpot, pot_out, npatch, args: SolverArgs, repeats=3):
+        """Time every layout of ghost size gs on this batch and keep the fastest: (ppw, pad, ms)."""
+        p, d, ms = C.c_int(), C.c_int(), C.c_float()
+        _check(load().pgamd_autotune(self.h, gs, _vp(rho), _vp(cpot), _vp(pot_out), npatch, C.byref(args), repeats,
+                                     C.byref(p), C.byref(d), C.byref(ms)))
+        return p.value, d.value, ms.value
+
+    def autotune_save(self, path):
+        _check(load().pgamd_autotune_save(self.h, path.encode()))
+
+    def autotune_load(self, path):
+        _check(load().pgamd_autotune_load(self.h, path.encode()))
+
     # --- mixed batches and graphs ---------------------------------------------------------------------------
     def plan(self, ghost_sizes, schemes=None) -> "Plan":
         return Plan(self, ghost_sizes, schemes)
@@ -345,3 +377,56 @@ def solve_host(rho, cpot, flu, *, gs, gg=1, scheme=SOR, init_mode=INIT_INTERP, t
         torch.cuda.synchronize(dev)
         return (d_out.cpu().numpy(), None if d_flu is None else d_flu.cpu().numpy(), d_it.cpu().numpy(),
                 d_st.cpu().numpy())
+
+
+class CNN:
+    """pgamd_cnn_*: the warm-start network (parameters on the device; BF16 tensors as torch.int16/uint16)."""
+
+    NPARAMS = 31617
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(load().pgamd_cnn_create(ctx.h, seed, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            _check(load().pgamd_cnn_destroy(self.h))
+            self.h = None
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.NPARAMS, np.float32)
+        _check(load().pgamd_cnn_get_params(self.h, p.ctypes.data_as(_FP), self.NPARAMS))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(load().pgamd_cnn_set_params(self.h, p.ctypes.data_as(_FP), p.size))
+
+    def save(self, path):
+        np.save(path, self.get_params())
+
+    def load_params(self, path):
+        self.set_params(np.load(path, allow_pickle=False))
+
+    def prepare(self, rho, cpot, npatch, gs, args: SolverArgs, x0, scale, stream=None):
+        _check(load().pgamd_cnn_prepare(self.ctx.h, _vp(rho), _vp(cpot), npatch, gs, C.byref(args), _vp(x0),
+                                        _vp(scale), stream))
+
+    def forward(self, x0, count, out, stream=None):
+        _check(load().pgamd_cnn_forward(self.h, _vp(x0), count, _vp(out), stream))
+
+    def loss_grad(self, x0, target, count, grad=None, stream=None) -> float:
+        loss = C.c_float()
+        _check(load().pgamd_cnn_loss_grad(self.h, _vp(x0), _vp(target), count, C.byref(loss), _vp(grad), stream))
+        return loss.value
+
+    def train_step(self, x0, target, count, lr, want_loss=False, stream=None):
+        loss = C.c_float()
+        _check(load().pgamd_cnn_train_step(self.h, _vp(x0), _vp(target), count, lr,
+                                           C.byref(loss) if want_loss else None, stream))
+        return loss.value if want_loss else None
+
+    def warm_start(self, rho, cpot, npatch, args: SolverArgs, guess, stream=None):
+        _check(load().pgamd_cnn_warm_start(self.ctx.h, self.h, _vp(rho), _vp(cpot), npatch, C.byref(args),
+                                           _vp(guess), stream))

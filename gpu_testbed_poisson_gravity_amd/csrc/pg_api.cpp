@@ -65,6 +65,7 @@ int validate_options(const pgamd_options& o) {
   if (o.term_mode < 0 || o.term_mode > 1) return fail(PGAMD_ERR_ARG, "term_mode %d", o.term_mode);
   if (o.check_every < 1) return fail(PGAMD_ERR_ARG, "check_every %d < 1", o.check_every);
   if (!(o.rel_tol >= 0.0f)) return fail(PGAMD_ERR_ARG, "rel_tol must be >= 0");
+  if (!(o.guess_fallback_tol >= 0.0f)) return fail(PGAMD_ERR_ARG, "guess_fallback_tol must be >= 0");
   return PGAMD_OK;
 }
 
@@ -105,6 +106,7 @@ SolveParams make_params(const pgamd_options& o, const pgamd_solver_args& a) {
   p.p5 = a.p5_gradient;
   // Gra_Const in double, rounded once (CPU_GravitySolver.cpp:23, CUAPI_Asyn_PoissonGravitySolver.cu:112-113)
   p.gra_const = (float)(a.p5_gradient ? (double)a.dt / (12.0 * (double)a.dh) : (double)a.dt / (2.0 * (double)a.dh));
+  p.guess_tol = o.guess_fallback_tol;
   return p;
 }
 
@@ -172,6 +174,7 @@ void pgamd_default_options(pgamd_options* o) {
   o->check_every = 1;
   o->rel_tol = 1e-5f;
   o->fuse_gravity = 1;
+  o->guess_fallback_tol = 0.0f;
 }
 
 const char* pgamd_last_error(void) { return g_last_error.c_str(); }

@@ -34,12 +34,16 @@ struct SolveParams {
   int fuse_gravity;
   int p5;
   float gra_const;
+  // initial-guess check (init_mode 2): a guess whose relative residual exceeds guess_tol is replaced by
+  // the zero guess before the first sweep; 0 disables the check
+  float guess_tol;
 };
 
 // Status bits written per patch.
 constexpr int kStatusConverged = 1;  // the stopping rule was met before max_iter
 constexpr int kStatusMaxIter = 2;    // max_iter reached
 constexpr int kStatusNonFinite = 4;  // a non-finite value is present in the returned potential
+constexpr int kStatusFallback = 8;   // the caller's guess failed the residual check; the zero guess was used
 
 // One batch of patches sharing a ghost size, with dense per-patch strides (the reference layout).
 struct UniformBatch {

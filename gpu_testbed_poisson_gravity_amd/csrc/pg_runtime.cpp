@@ -353,3 +353,120 @@ int pgamd_graph_destroy(pgamd_graph* g) {
 }
 
 }  // extern "C"
+
+// ---- autotuner ------------------------------------------------------------------------------------------
+//
+// Search space per ghost size: patches per wavefront in {1, 2, 4} (where they fit the 64 lanes) x row
+// padding in {0..3} floats.  Each candidate runs one untimed solve and `repeats` timed ones on the caller's
+// stream-less default queue; the fastest mean wins.  The table is cached as a small JSON document:
+//   {"device": "<name>", "layouts": {"1": [ppw, pad], ..., "5": [ppw, pad]}}
+
+#include <cstdio>
+#include <string>
+
+extern "C" {
+
+int pgamd_autotune(pgamd_context* c, int gs, const float* rho, const float* cpot, float* pot_out, int npatch,
+                   const pgamd_solver_args* a, int repeats, int* best_ppw, int* best_pad, float* best_ms) {
+  if (!c || !a || !rho || !cpot || !pot_out) return set_error(PGAMD_ERR_ARG, "null argument");
+  if (gs < 1 || gs > kMaxGhost || npatch <= 0 || repeats < 1) return set_error(PGAMD_ERR_ARG, "bad autotune size");
+  if (hipSetDevice(ctx_device(c)) != hipSuccess) return set_error(PGAMD_ERR_DEVICE, "hipSetDevice failed");
+  pgamd_options o = ctx_options(c);
+  o.pot_ghost_size = gs;
+  o.pot_scheme = PGAMD_SOR;
+  o.fuse_gravity = 0;
+  pgamd_solver_args at = *a;
+  at.graacc = 0;
+  at.poisson = 1;
+  int rc = validate_args(o, at, npatch);
+  if (rc != PGAMD_OK) return rc;
+  const SolveParams p = make_params(o, at);
+  hipStream_t st = ctx_stream(c);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return set_error(PGAMD_ERR_DEVICE, "hipEventCreate failed");
+  SorLayout& lay = ctx_layout(c);
+  int bp = lay.ppw[gs], bd = lay.pad[gs];
+  float bt = 1e30f;
+  UniformBatch b{rho, cpot, nullptr, pot_out, nullptr, nullptr, nullptr, npatch, gs, PGAMD_SOR};
+  for (int ppw : {1, 2, 4}) {
+    if (!supported_ppw(gs, ppw)) continue;
+    for (int pad = 0; pad <= 3; pad++) {
+      hipError_t e = launch_sor_uniform_dispatch(b, p, ppw, pad, st);
+      if (e != hipSuccess) continue;  // layout does not launch on this device (e.g. LDS): skip it
+      hipEventRecord(e0, st);
+      for (int r = 0; r < repeats; r++) launch_sor_uniform_dispatch(b, p, ppw, pad, st);
+      hipEventRecord(e1, st);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return set_error(PGAMD_ERR_DEVICE, "autotune synchronisation failed");
+      }
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, e0, e1);
+      ms /= repeats;
+      if (ms < bt) {
+        bt = ms;
+        bp = ppw;
+        bd = pad;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  lay.ppw[gs] = bp;
+  lay.pad[gs] = bd;
+  if (best_ppw) *best_ppw = bp;
+  if (best_pad) *best_pad = bd;
+  if (best_ms) *best_ms = bt;
+  return PGAMD_OK;
+}
+
+int pgamd_autotune_save(pgamd_context* c, const char* path) {
+  if (!c || !path) return set_error(PGAMD_ERR_ARG, "null argument");
+  char name[256];
+  pgamd_device_info(c, name, sizeof(name));
+  FILE* f = fopen(path, "w");
+  if (!f) return set_error(PGAMD_ERR_ARG, "cannot open the autotune cache for writing");
+  const SorLayout& lay = ctx_layout(c);
+  std::string dev(name);
+  dev = dev.substr(0, dev.find(';'));
+  fprintf(f, "{\"device\": \"%s\", \"layouts\": {", dev.c_str());
+  for (int g = 1; g <= kMaxGhost; g++) fprintf(f, "%s\"%d\": [%d, %d]", g > 1 ? ", " : "", g, lay.ppw[g], lay.pad[g]);
+  fprintf(f, "}}\n");
+  fclose(f);
+  return PGAMD_OK;
+}
+
+int pgamd_autotune_load(pgamd_context* c, const char* path) {
+  if (!c || !path) return set_error(PGAMD_ERR_ARG, "null argument");
+  FILE* f = fopen(path, "r");
+  if (!f) return set_error(PGAMD_ERR_ARG, "cannot open the autotune cache");
+  std::string s;
+  char buf[512];
+  size_t n;
+  while ((n = fread(buf, 1, sizeof(buf), f)) > 0) s.append(buf, n);
+  fclose(f);
+  char name[256];
+  pgamd_device_info(c, name, sizeof(name));
+  std::string dev(name);
+  dev = dev.substr(0, dev.find(';'));
+  if (s.find("\"device\": \"" + dev + "\"") == std::string::npos)
+    return set_error(PGAMD_ERR_STATE, "autotune cache was written for another device");
+  int ppw[6], pad[6];
+  for (int g = 1; g <= kMaxGhost; g++) {
+    const std::string key = "\"" + std::to_string(g) + "\": [";
+    const size_t k = s.find(key);
+    if (k == std::string::npos || sscanf(s.c_str() + k + key.size(), "%d, %d", &ppw[g], &pad[g]) != 2 ||
+        !supported_ppw(g, ppw[g]) || pad[g] < 0 || pad[g] > 3)
+      return set_error(PGAMD_ERR_ARG, "malformed autotune cache");
+  }
+  SorLayout& lay = ctx_layout(c);
+  for (int g = 1; g <= kMaxGhost; g++) {
+    lay.ppw[g] = ppw[g];
+    lay.pad[g] = pad[g];
+  }
+  return PGAMD_OK;
+}
+
+}  // extern "C"

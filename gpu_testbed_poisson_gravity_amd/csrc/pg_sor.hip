@@ -309,13 +309,42 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
     }
   __syncthreads();
 
+  // B'. reject a caller guess whose relative residual exceeds guess_tol: restart from the zero guess
+  float res[2 * R];
+  bool fell_back = false;
+  if (P.init_mode == 2 && P.guess_tol > 0.0f) {
+    float num = half_sweep<GS, 1, 1, true>(L, lds, lb, 0.0f, res);
+    num += half_sweep<GS, 0, 1, true>(L, lds, lb, 0.0f, res);
+    float den = 0.0f;
+#pragma unroll
+    for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+      for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+        for (int zi = 0; zi < R; zi++) den += fabsf(L.phi[yy][xx][zi]);
+    if (!active) num = den = 0.0f;
+    num = seg_sum<SEG>(num);
+    den = seg_sum<SEG>(den);
+    fell_back = lp.valid && !(num <= P.guess_tol * den);
+    if (fell_back) {
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+          for (int zi = 0; zi < R; zi++) L.phi[yy][xx][zi] = 0.0f;
+    }
+    store_colour<GS, 1>(L, lds, lb, active && fell_back);
+    store_colour<GS, 0>(L, lds, lb, active && fell_back);
+    __syncthreads();
+  }
+
   // C. red-black iterations
   bool running = active;
   int it_done = P.max_iter, st = kStatusMaxIter;
   const bool relres = P.term_mode == 1, stall = P.term_mode == 0;
   const int first_check = P.min_iter > 1 ? P.min_iter : 1;
   float sum_b = 0.0f, old = FLT_MAX;
-  float res[2 * R];
   for (int t = 0; t < P.max_iter; ++t) {
     const bool check = relres && t >= first_check && (t - first_check) % P.check_every == 0;
     const bool pre_check = relres && t + 1 >= first_check && (t + 1 - first_check) % P.check_every == 0;
@@ -377,8 +406,10 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
 #pragma unroll
       for (int zi = 0; zi < R; zi++) bad |= !isfinite(L.phi[yy][xx][zi]);
   const uint64_t bad_mask = __ballot(active && bad);
-  const uint64_t seg_mask = SEG == 64 ? ~0ull : (((1ull << SEG) - 1) << (slot * SEG));
+  uint64_t seg_mask = ~0ull;
+  if constexpr (SEG < 64) seg_mask = ((1ull << SEG) - 1) << (slot * SEG);
   if (bad_mask & seg_mask) st |= kStatusNonFinite;
+  if (fell_back) st |= kStatusFallback;
   if (lp.valid && pl == 0) {
     if (lp.iters) *lp.iters = it_done;
     if (lp.status) *lp.status = st;

@@ -54,6 +54,7 @@ extern "C" {
 #define PGAMD_STATUS_CONVERGED 1
 #define PGAMD_STATUS_MAX_ITER 2
 #define PGAMD_STATUS_NONFINITE 4
+#define PGAMD_STATUS_FALLBACK 8   /* init GUESS: guess rejected (relative residual > guess_fallback_tol) */
 
 typedef struct pgamd_context pgamd_context;
 
@@ -67,6 +68,8 @@ typedef struct {
   int check_every;      /* >= 1                                                    */
   float rel_tol;        /* RELRES tolerance                                        */
   int fuse_gravity;     /* 1: gravity update inside the solve kernel, phi still in LDS */
+  float guess_fallback_tol; /* init GUESS, SOR: use the zero guess instead when the guess's relative
+                               residual exceeds this (0: never) */
 } pgamd_options;
 
 /* Fills the defaults: ghost 5/1, SOR, interpolated guess, reference rule, check_every 1, rel_tol 1e-5,
@@ -116,6 +119,7 @@ int pgamd_get_patch_stats(pgamd_context* ctx, int* iters, int* status, int npatc
 int pgamd_set_layout(pgamd_context* ctx, int ghost_size, int patches_per_wave, int row_pad);
 int pgamd_get_layout(pgamd_context* ctx, int ghost_size, int* patches_per_wave, int* row_pad);
 
+
 /* ---- device-resident entry points (inputs already in HBM; no reference counterpart: the reference
  * always stages through host memory).  All pointers are device pointers; `stream` is a hipStream_t
  * (NULL = the HIP default stream).  iters / status may be NULL. */
@@ -134,6 +138,16 @@ typedef struct {
 int pgamd_solve_device(pgamd_context* ctx, const float* d_rho, const float* d_pot_in, const float* d_guess,
                        float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* args, int* d_iters,
                        int* d_status, void* stream);
+
+/* Autotuner: times every supported layout of one ghost size on the caller's device batch (SOR with the
+ * context's options, gravity off; repeats >= 1 timed solves per candidate after one warm-up), keeps the
+ * fastest in the context and returns it.  d_pot_out is overwritten.  The layout table can be saved and
+ * reloaded as JSON, keyed by device name. */
+int pgamd_autotune(pgamd_context* ctx, int ghost_size, const float* d_rho, const float* d_pot_in, float* d_pot_out,
+                   int npatch, const pgamd_solver_args* args, int repeats, int* best_ppw, int* best_pad,
+                   float* best_ms);
+int pgamd_autotune_save(pgamd_context* ctx, const char* path);
+int pgamd_autotune_load(pgamd_context* ctx, const char* path);
 
 /* ---- mixed batches (no reference counterpart: the reference compiles one ghost size and one scheme per
  * binary).  A plan fixes the ghost size (1..5) and scheme (SOR/MG) of every patch, the packed layout of
@@ -162,6 +176,35 @@ int pgamd_graph_capture_device(pgamd_context* ctx, const float* d_rho, const flo
                                int* d_iters, int* d_status, pgamd_graph** graph);
 int pgamd_graph_launch(pgamd_graph* graph, void* stream);
 int pgamd_graph_destroy(pgamd_graph* graph);
+
+/* ---- learned warm start (no reference counterpart) ----------------------------------------------------
+ * A 3D CNN (BF16 weights, FP32 accumulation, hand-written MFMA convolutions) predicts the patch-proper 8^3
+ * part of the converged potential from an 18^3 input with 8 BF16 channels per voxel ([n][18][18][18][8]):
+ * channel 0 = interpolated coarse potential / s, channel 1 = Poi_Coeff*dh^2*rho / s on the solve interior,
+ * channels 2..7 = 0, where s = max|interpolated potential| of the patch (smaller cubes are centred with
+ * zeros around).  The prediction d ([n][8^3], FP32) gives the guess interp + s*d on the patch proper and
+ * the interpolated values elsewhere; solve with init_mode PGAMD_INIT_GUESS and, for a safe fallback,
+ * guess_fallback_tol > 0.  Parameters: 31617 FP32 values (layer order W1 b1 W2 b2 ... W5 b5 w6 b6). */
+typedef struct pgamd_cnn pgamd_cnn;
+int pgamd_cnn_num_params(void);
+int pgamd_cnn_create(pgamd_context* ctx, uint64_t seed, pgamd_cnn** net);
+int pgamd_cnn_destroy(pgamd_cnn* net);
+int pgamd_cnn_get_params(pgamd_cnn* net, float* host_params, int count);
+int pgamd_cnn_set_params(pgamd_cnn* net, const float* host_params, int count);
+/* Network input of a batch of same-ghost-size patches (d_x0: [n][18^3][8] BF16 bits, d_scale: [n]). */
+int pgamd_cnn_prepare(pgamd_context* ctx, const float* d_rho, const float* d_pot_in, int npatch, int ghost_size,
+                      const pgamd_solver_args* args, uint16_t* d_x0, float* d_scale, void* stream);
+int pgamd_cnn_forward(pgamd_cnn* net, const uint16_t* d_x0, int count, float* d_out, void* stream);
+/* Mean-squared-error loss against d_target ([n][8^3]) and its gradient (FP32, 31617 values, may be NULL);
+ * *loss (may be NULL) is read back synchronously. */
+int pgamd_cnn_loss_grad(pgamd_cnn* net, const uint16_t* d_x0, const float* d_target, int count, float* loss,
+                        float* d_grad, void* stream);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on a batch. */
+int pgamd_cnn_train_step(pgamd_cnn* net, const uint16_t* d_x0, const float* d_target, int count, float lr,
+                         float* loss, void* stream);
+/* Guess of the interior ([n][R^3]) for a batch at the context's ghost size: prepare, forward, assemble. */
+int pgamd_cnn_warm_start(pgamd_context* ctx, pgamd_cnn* net, const float* d_rho, const float* d_pot_in, int npatch,
+                         const pgamd_solver_args* args, float* d_guess, void* stream);
 
 #ifdef __cplusplus
 }
