@@ -23,7 +23,13 @@ sys.path.insert(0, ROOT)
 METRIC = "patch-solves/s at rel. residual ≤1e-5: warm-start vs plain SOR, 1 MI355X"
 HBM_PEAK = 8.0e12          # B/s, MI355X published
 LDS_PEAK = 256 * 128 * 2.4e9  # B/s: 256 CUs x 128 B/clk x 2.4 GHz
+MFMA_BF16_PEAK = 2.5e15    # FLOP/s, dense BF16, MI355X published
 LDS_BYTES_PER_UPDATE = 36  # 7 phi reads + 1 rho read + 1 write, FP32
+# Warm-start network FLOPs per patch: 16^3 x 16 x 216 x 2 (layer 1) + (14^3+12^3+10^3+8^3) x 16 x 432 x 2
+# (layers 2-5) on the matrix cores, + 8^3 x 432 x 2 (output layer, VALU)
+CNN_MFMA_FLOP = 16 ** 3 * 16 * 216 * 2 + (14 ** 3 + 12 ** 3 + 10 ** 3 + 8 ** 3) * 16 * 432 * 2
+CNN_FLOP = CNN_MFMA_FLOP + 512 * 432 * 2
+WEIGHTS = os.path.join(ROOT, "gpu_testbed_poisson_gravity_amd", "weights", "warm_start_gs5.npy")
 
 
 def log(*a):
@@ -42,6 +48,10 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--pad", type=int, default=None, help="SOR split-row padding (default: library choice)")
     ap.add_argument("--ppw", type=int, default=None, help="patches per wavefront (default: library choice)")
+    ap.add_argument("--weights", default=WEIGHTS, help="trained warm-start network (.npy)")
+    ap.add_argument("--fallback-tol", type=float, default=1e-2,
+                    help="guess check: a network guess with relative residual above this restarts from zero")
+    ap.add_argument("--mode", choices=["both", "plain", "warm"], default="both")
     return ap.parse_args()
 
 
@@ -60,54 +70,97 @@ def main():
     rho, cpot, flu, _ = I.plummer(a.npatch, a.gs, seed=a.seed, energy=1.0)
     log(f"[bench] inputs generated in {time.time() - t0:.1f}s")
     g = pg.geometry(a.gs)
+    R = g["R"]
     d_rho = torch.from_numpy(rho).to(dev)
     d_cpot = torch.from_numpy(cpot).to(dev)
     d_flu0 = torch.from_numpy(flu).to(dev)
     d_flu = d_flu0.clone()
     d_out = torch.empty((a.npatch, 10, 10, 10), device=dev)
+    d_guess = torch.empty((a.npatch, R, R, R), device=dev)
     d_it = torch.zeros(a.npatch, dtype=torch.int32, device=dev)
     d_st = torch.zeros(a.npatch, dtype=torch.int32, device=dev)
     poi = 4.0 * np.pi * 1e-5
     args = pg.solver_args(dt=1e-3, sor_min_iter=10, sor_max_iter=1000, sor_omega=1.8, poi_coeff=poi)
     ctx = pg.Context(0)
-    ctx.set_options(pot_ghost_size=a.gs, gra_ghost_size=1, pot_scheme=pg.SOR, init_mode=pg.INIT_ZERO,
-                    term_mode=pg.TERM_RELRES, check_every=a.check_every, rel_tol=1e-5, fuse_gravity=1)
     if a.pad is not None or a.ppw is not None:
         ppw0, pad0 = ctx.get_layout(a.gs)
         ctx.set_layout(a.gs, a.ppw if a.ppw is not None else ppw0, a.pad if a.pad is not None else pad0)
     stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
 
-    def step():
-        d_flu.copy_(d_flu0)  # restore the fluid so every step does identical work (tiny D2D copy)
-        ctx.solve_device(d_rho, d_cpot, d_out, d_flu, a.npatch, args, iters=d_it, status=d_st,
-                         stream=stream.cuda_stream)
+    def run(init_mode, net):
+        """warmup + timed steps; events bracket the network phase and the solve separately."""
+        ctx.set_options(pot_ghost_size=a.gs, gra_ghost_size=1, pot_scheme=pg.SOR, init_mode=init_mode,
+                        term_mode=pg.TERM_RELRES, check_every=a.check_every, rel_tol=1e-5, fuse_gravity=1,
+                        guess_fallback_tol=a.fallback_tol if net is not None else 0.0)
 
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize(dev)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * a.steps)]
-    torch.cuda.synchronize(dev)
-    t_wall = time.perf_counter()
-    for s in range(a.steps):
-        d_flu.copy_(d_flu0)
-        ev[2 * s].record(stream)
-        ctx.solve_device(d_rho, d_cpot, d_out, d_flu, a.npatch, args, iters=d_it, status=d_st,
-                         stream=stream.cuda_stream)
-        ev[2 * s + 1].record(stream)
-    torch.cuda.synchronize(dev)
-    wall = time.perf_counter() - t_wall
-    kern_ms = [ev[2 * s].elapsed_time(ev[2 * s + 1]) for s in range(a.steps)]
-    ms = float(np.mean(kern_ms))
-    iters = d_it.cpu().numpy()
-    status = d_st.cpu().numpy()
-    value = a.npatch / (ms * 1e-3)
-    R = g["R"]
-    upd = float(iters.astype(np.float64).sum()) * R ** 3
-    lds_alg = upd * LDS_BYTES_PER_UPDATE
-    hbm_alg = a.npatch * 4.0 * (R ** 3 + g["C"] ** 3 + 10 ** 3 + 4 * 512 + 5 * 512)
-    roof = {"bound": "lds", "achieved": lds_alg / (ms * 1e-3), "peak": LDS_PEAK, "unit": "B/s",
-            "frac": lds_alg / (ms * 1e-3) / LDS_PEAK, "traffic": None,
-            "hbm_achieved": hbm_alg / (ms * 1e-3), "hbm_peak": HBM_PEAK}
+        def step(e=None):
+            d_flu.copy_(d_flu0)  # restore the fluid so every step does identical work (D2D, outside events)
+            if e is not None:
+                e[0].record(stream)
+            if net is not None:
+                net.warm_start(d_rho, d_cpot, a.npatch, args, d_guess, stream=sp)
+            if e is not None:
+                e[1].record(stream)
+            ctx.solve_device(d_rho, d_cpot, d_out, d_flu, a.npatch, args, guess=d_guess if net is not None else None,
+                             iters=d_it, status=d_st, stream=sp)
+            if e is not None:
+                e[2].record(stream)
+
+        for _ in range(a.warmup):
+            step()
+        torch.cuda.synchronize(dev)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
+        t_wall = time.perf_counter()
+        for s in range(a.steps):
+            step(evs[s])
+        torch.cuda.synchronize(dev)
+        wall = time.perf_counter() - t_wall
+        pre = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+        sol = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+        it, st = d_it.cpu().numpy(), d_st.cpu().numpy()
+        return {"value": a.npatch / ((pre + sol) * 1e-3), "ms_per_step": pre + sol, "network_ms": pre,
+                "solve_ms": sol, "wall_ms_per_step": wall / a.steps * 1e3, "mean_iters": float(it.mean()),
+                "p99_iters": float(np.percentile(it, 99)), "max_iters": int(it.max()),
+                "converged": int(((st & pg.STATUS_CONVERGED) != 0).sum()),
+                "fallbacks": int(((st & pg.STATUS_FALLBACK) != 0).sum()), "iters": it}
+
+    def lds_roof(r):
+        alg = float(r["iters"].astype(np.float64).sum()) * R ** 3 * LDS_BYTES_PER_UPDATE
+        ach = alg / (r["solve_ms"] * 1e-3)
+        hbm = a.npatch * 4.0 * (R ** 3 + g["C"] ** 3 + 10 ** 3 + 4 * 512 + 5 * 512)
+        return {"kernel": "sor_uniform_kernel", "bound": "lds", "achieved": ach / 1e9, "peak": LDS_PEAK / 1e9,
+                "unit": "GB/s", "frac": ach / LDS_PEAK, "traffic": None,
+                "algorithmic_bytes_per_launch": alg, "launch_ms": r["solve_ms"],
+                "hbm_achieved": hbm / (r["solve_ms"] * 1e-3) / 1e9, "hbm_peak": HBM_PEAK / 1e9}
+
+    plain = warm = None
+    if a.mode in ("both", "plain"):
+        plain = run(pg.INIT_ZERO, None)
+        log(f"[bench] plain: {plain['value']:.4g} patch-solves/s, {plain['mean_iters']:.1f} iterations")
+    net = None
+    if a.mode in ("both", "warm"):
+        if os.path.exists(a.weights):
+            net = pg.CNN(ctx)
+            net.load_params(a.weights)
+            warm = run(pg.INIT_GUESS, net)
+            log(f"[bench] warm: {warm['value']:.4g} patch-solves/s, {warm['mean_iters']:.1f} iterations, "
+                f"network {warm['network_ms']:.3f} ms + solve {warm['solve_ms']:.3f} ms")
+        else:
+            log(f"[bench] no trained weights at {a.weights}: warm-start leg skipped")
+    head_r = warm if warm is not None else plain
+    roof_sor = lds_roof(head_r)
+    roofs = {"sor": roof_sor}
+    roof = roof_sor
+    if warm is not None:
+        flop = CNN_MFMA_FLOP * a.npatch
+        ach = flop / (warm["network_ms"] * 1e-3)
+        roofs["cnn"] = {"kernel": "warm-start network (prep, 5 MFMA convolutions, output layer, guess)",
+                        "bound": "mfma", "achieved": ach / 1e12, "peak": MFMA_BF16_PEAK / 1e12, "unit": "TFLOP/s",
+                        "frac": ach / MFMA_BF16_PEAK, "traffic": None, "algorithmic_flop_per_launch": flop,
+                        "launch_ms": warm["network_ms"]}
+        if warm["network_ms"] > warm["solve_ms"]:
+            roof = roofs["cnn"]
 
     cpu = None
     if a.cpu_sample > 0:
@@ -121,22 +174,29 @@ def main():
         _, _, it_cpu, _ = O.solve(rho[:n], cpot[:n], flu[:n], p, nthreads=nt)
         dt = time.perf_counter() - t
         cpu = {"value": n / dt, "unit": "patch-solves/s", "cores": nt, "kind": "port",
-               "sample": f"first {n} of the {a.npatch} patches, FP32 oracle, OpenMP over patches",
+               "sample": f"first {n} of the {a.npatch} patches, zero guess, FP32 oracle, OpenMP over patches",
                "mean_iters": float(it_cpu.mean())}
 
+    def strip(r):
+        return None if r is None else {k: v for k, v in r.items() if k != "iters"}
+
     out = {
-        "metric": METRIC, "value": value, "unit": "patch-solves/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{a.npatch} patches gs={a.gs} (18^3) Plummer 1-8 spheres, SOR omega=1.8, "
-                               f"rel.res 1e-5, min10/max1000, zero guess, fused gravity",
+        "metric": METRIC, "value": head_r["value"], "unit": "patch-solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": head_r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{a.npatch} patches gs={a.gs} (18^3 cube), Plummer 1-8 spheres, SOR omega=1.8, "
+                               f"rel. residual 1e-5, min 10 / max 1000 iterations, fused gravity; value = "
+                               + ("learned warm start (guess check 1e-2)" if warm is not None else
+                                  "zero guess (no trained weights)"),
                    "npatch": a.npatch, "gs": a.gs, "check_every": a.check_every},
-        "plain": {"value": value, "ms_per_step": ms, "mean_iters": float(iters.mean()),
-                  "max_iters": int(iters.max()), "converged": int((status == 1).sum())},
-        "wall_ms_per_step": wall / a.steps * 1e3,
-        "roofline": roof, "cpu_baseline": cpu,
+        "plain": strip(plain), "warm": strip(warm),
+        "warm_vs_plain": (warm["value"] / plain["value"]) if (warm is not None and plain is not None) else None,
+        "wall_ms_per_step": head_r["wall_ms_per_step"],
+        "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    if net is not None:
+        net.close()
     ctx.close()
 
 

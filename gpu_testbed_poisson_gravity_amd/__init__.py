@@ -380,9 +380,13 @@ def solve_host(rho, cpot, flu, *, gs, gg=1, scheme=SOR, init_mode=INIT_INTERP, t
 
 
 class CNN:
-    """pgamd_cnn_*: the warm-start network (parameters on the device; BF16 tensors as torch.int16/uint16)."""
+    """pgamd_cnn_*: the warm-start network (parameters on the device; BF16 tensors as torch.int16).
 
-    NPARAMS = 31617
+    Packed input x0: [n, 10^3*16] BF16 bits; output / target: [n, 512, 8] FP32."""
+
+    NPARAMS = 41568
+    INPUT = 10 ** 3 * 16
+    OUTPUTS = 512 * 8
 
     def __init__(self, ctx: Context, seed: int = 0):
         self.ctx = ctx

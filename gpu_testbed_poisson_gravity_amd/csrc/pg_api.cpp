@@ -56,7 +56,9 @@ int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
 int validate_options(const pgamd_options& o) {
   if (o.pot_ghost_size < 1 || o.pot_ghost_size > kMaxGhost)
     return fail(PGAMD_ERR_ARG, "pot_ghost_size %d outside [1,5]", o.pot_ghost_size);
-  if (o.gra_ghost_size < 1 || o.gra_ghost_size > 2 || o.gra_ghost_size > o.pot_ghost_size)
+  // GRA_GHOST_SIZE is 1 or 2 for the gravity stencil (the reference's range); a Poisson-only call may ask
+  // for a wider output window, up to the whole solve cube (gra_ghost_size = pot_ghost_size).
+  if (o.gra_ghost_size < 1 || o.gra_ghost_size > o.pot_ghost_size)
     return fail(PGAMD_ERR_ARG, "gra_ghost_size %d invalid for pot_ghost_size %d", o.gra_ghost_size,
                 o.pot_ghost_size);
   if (o.pot_scheme != PGAMD_SOR && o.pot_scheme != PGAMD_MG)
@@ -74,6 +76,8 @@ int validate_args(const pgamd_options& o, const pgamd_solver_args& a, int npatch
   if (npatch < 0) return fail(PGAMD_ERR_ARG, "negative patch count %d", npatch);
   if (a.poisson && a.int_scheme != PGAMD_INT_CQUAD && a.int_scheme != PGAMD_INT_QUAD)
     return fail(PGAMD_ERR_ARG, "incorrect parameter IntScheme = %d", a.int_scheme);
+  if (a.graacc && o.gra_ghost_size > 2)
+    return fail(PGAMD_ERR_ARG, "the gravity update needs GRA_GHOST_SIZE <= 2 (wider windows are Poisson-only)");
   if (a.graacc && a.p5_gradient && o.gra_ghost_size < 2)
     return fail(PGAMD_ERR_ARG, "the 5-point gradient needs GRA_GHOST_SIZE = 2");
   if (a.poisson && o.pot_scheme == PGAMD_SOR && (a.sor_max_iter < 0 || a.sor_min_iter < 0))

@@ -1,4 +1,5 @@
-// Launchers of the warm-start CNN kernels (pg_cnn.hip).  BF16 tensors are passed as uint16_t bit patterns.
+// Launchers and sizes of the warm-start CNN kernels (pg_cnn.hip).  BF16 tensors are passed as uint16_t bit
+// patterns.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,25 +9,33 @@
 
 namespace pgamd {
 
-hipError_t cnn_conv_fwd(const uint16_t* X, const uint16_t* W, const float* b, uint16_t* Y, uint16_t* H, int n, int S,
-                        int cin, bool resid, hipStream_t s);
-hipError_t cnn_conv6_fwd(const uint16_t* Y5, const float* w6, const float* b6, float* D, int n, hipStream_t s);
-hipError_t cnn_mse(const float* D, const float* T, float* dD, float* partial, int count, int nblocks, hipStream_t s);
-hipError_t cnn_conv6_bwd(const uint16_t* Y5, const float* w6, const float* dD, float* dY5, float* gpart, int n,
-                         hipStream_t s);
+constexpr int kCnnEdge = 8;                                  // edge of every layer's output (coarse 2h grid)
+constexpr int kCnnPadded = kCnnEdge + 2;                     // padded input edge
+constexpr int kCnnChannels = 16;
+constexpr int kCnnOut = 8;                                   // output channels = 2^3 sub-cells
+constexpr int kCnnLayers = 6;
+constexpr int kCnnK = 27 * kCnnChannels;                     // 432
+constexpr int kCnnLayerParams = kCnnChannels * kCnnK + kCnnChannels;   // W [16][432] + b [16]
+constexpr int kCnnParams = kCnnLayers * kCnnLayerParams;     // 41568
+constexpr int kCnnVox = kCnnEdge * kCnnEdge * kCnnEdge;      // 512
+constexpr int kCnnPadVox = kCnnPadded * kCnnPadded * kCnnPadded;   // 1000
+constexpr int kCnnInput = kCnnPadVox * kCnnChannels;         // bf16 values per packed input
+constexpr int kCnnOutputs = kCnnVox * kCnnOut;               // fp32 values per output
+
+// mode 0: first layer, 1: residual layer, 2: linear output layer (D)
+hipError_t cnn_conv_fwd(const uint16_t* X, const uint16_t* W, const float* b, uint16_t* Y, uint16_t* H, float* D,
+                        int n, int mode, hipStream_t s);
+hipError_t cnn_mse(const float* D, const float* T, float* dY, float* partial, int nvox, int nblocks, hipStream_t s);
 hipError_t cnn_relu_bwd(const float* dY, const uint16_t* H, uint16_t* dH, float* bpart, size_t count, int nblocks,
                         hipStream_t s);
-hipError_t cnn_conv_bwd_data(const uint16_t* dH, const uint16_t* WT, const float* dYres, float* dX, int n, int S,
+hipError_t cnn_conv_bwd_data(const uint16_t* dH, const uint16_t* WT, const float* dYres, float* dX, int n,
                              hipStream_t s);
-int cnn_bwd_weight_blocks(int n, int S);
-hipError_t cnn_conv_bwd_weight(const uint16_t* X, const uint16_t* dH, float* wpart, int n, int S, int cin,
-                               hipStream_t s);
+int cnn_bwd_weight_blocks(int n);
+hipError_t cnn_conv_bwd_weight(const uint16_t* X, const uint16_t* dH, float* wpart, int n, hipStream_t s);
 hipError_t cnn_reduce_parts(const float* part, int parts, int len, float* dst, hipStream_t s);
 hipError_t cnn_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                     float c1, float c2, hipStream_t s);
-hipError_t cnn_to_bf16(const float* src, uint16_t* dst, int n, hipStream_t s);
-hipError_t cnn_transpose_w(const float* W, uint16_t* WT, hipStream_t s);
-hipError_t cnn_round_bf16(const float* src, float* dst, int n, hipStream_t s);
+hipError_t cnn_weight_copies(const float* P, uint16_t* Wb, uint16_t* WT, hipStream_t s);
 hipError_t cnn_prep(const float* rho, const float* cpot, int n, int gs, float cst, int int_scheme, uint16_t* x,
                     float* scale, hipStream_t s);
 hipError_t cnn_guess(const float* cpot, const float* D, const float* scale, int n, int gs, int int_scheme,

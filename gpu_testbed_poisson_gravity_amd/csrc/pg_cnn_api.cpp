@@ -1,13 +1,14 @@
 // C-ABI of the warm-start CNN (include/pg_amd.h, pgamd_cnn_*): parameters, training step (forward, MSE,
 // backward, Adam) and the inference pipeline that turns a batch of patches into an interior guess.
 //
-// Parameter vector (FP32 master copy), in this order:
-//   W1 [16][27*8]  b1[16]   W2..W5 [16][27*16]  b2..b5[16]   w6 [27*16]  b6[1]        (31617 values)
-// W* are indexed [co][tap*CIN + ci] with tap = dz*9 + dy*3 + dx.  Forward passes use BF16 copies of W1..W5
-// (and of w6, rounded to BF16 values); gradients are reduced in a fixed order (bitwise reproducible).
+// Parameter vector (FP32 master copy): for each of the 6 layers W [16][432] then b [16] (41568 values);
+// W[co][tap*16 + ci] with tap = dz*9 + dy*3 + dx.  Rows 8..15 of the output layer are unused.  Forward and
+// input-gradient passes use BF16 copies of the weights; gradients are reduced in a fixed order (bitwise
+// reproducible).
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,16 +24,12 @@ using namespace pgamd;
 
 namespace {
 
-constexpr int kCin1 = 8;
-constexpr int kK1 = 27 * kCin1;   // 216
-constexpr int kK = 27 * 16;       // 432
-constexpr int oW[6] = {0, 0, 3472, 10400, 17328, 24256};   // oW[l], l = 1..5
-constexpr int oB[6] = {0, 3456, 10384, 17312, 24240, 31168};
-constexpr int oW6 = 31184, oB6 = 31616, kNParams = 31617;
-constexpr int kSin[6] = {0, 18, 16, 14, 12, 10};           // input edge of layer l
+constexpr int kW = kCnnChannels * kCnnK;   // weights per layer
 constexpr int kMseBlocks = 64;
+constexpr int kMaxReduceBlocks = 4096;
 
-int sz3(int s) { return s * s * s; }
+int ow(int l) { return (l - 1) * kCnnLayerParams; }   // layer l = 1..6
+int ob(int l) { return ow(l) + kW; }
 
 }  // namespace
 
@@ -40,16 +37,15 @@ struct pgamd_cnn {
   int device = 0;
   int step = 0;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  uint16_t* wb = nullptr;   // bf16 W1 | W2 | W3 | W4 | W5 (forward layout)
-  uint16_t* wt = nullptr;   // bf16 transposed W2..W5
-  float* w6r = nullptr;     // w6 rounded to bf16 values, then b6
+  uint16_t* wb = nullptr;   // BF16 W1..W6 as stored
+  uint16_t* wt = nullptr;   // BF16 W1..W6 transposed per tap (input-gradient pass)
   // workspace
   int cap = 0;              // samples
   bool train_ws = false;
-  uint16_t* y[6] = {};      // y1..y5 (y[0] unused: input)
-  uint16_t* h[6] = {};      // pre-activations (training only)
-  uint16_t* dh = nullptr;
-  float *d = nullptr, *dd = nullptr, *dyA = nullptr, *dyB = nullptr, *part = nullptr, *lossp = nullptr;
+  uint16_t* y[6] = {};      // y1..y5, padded [n][10^3][16] with zero borders (y[0] unused: the input)
+  uint16_t* h[6] = {};      // pre-activations h1..h5 [n][8^3][16] (training only)
+  uint16_t* dh = nullptr;   // [n][8^3][16]
+  float *d = nullptr, *dy6 = nullptr, *dyA = nullptr, *dyB = nullptr, *part = nullptr, *lossp = nullptr;
   size_t part_floats = 0;
   // inference
   int icap = 0;
@@ -67,36 +63,38 @@ void free_ws(pgamd_cnn* n) {
   }
   hipFree(n->dh);
   hipFree(n->d);
-  hipFree(n->dd);
+  hipFree(n->dy6);
   hipFree(n->dyA);
   hipFree(n->dyB);
   hipFree(n->part);
   hipFree(n->lossp);
   n->dh = nullptr;
-  n->d = n->dd = n->dyA = n->dyB = n->part = n->lossp = nullptr;
+  n->d = n->dy6 = n->dyA = n->dyB = n->part = n->lossp = nullptr;
   n->cap = 0;
+  n->train_ws = false;
 }
+
+int reduce_blocks(size_t elems) { return (int)std::min<size_t>((elems + 255) / 256, kMaxReduceBlocks); }
 
 int ensure_ws(pgamd_cnn* n, int count, bool train) {
   if (count <= n->cap && (!train || n->train_ws)) return PGAMD_OK;
   free_ws(n);
   const size_t c = (size_t)count;
+  const size_t act = c * kCnnVox * kCnnChannels;
   bool ok = true;
   for (int l = 1; l <= 5; l++) {
-    const size_t e = c * sz3(kSin[l] - 2) * 16;
+    const size_t e = c * kCnnInput;
     ok &= hipMalloc(&n->y[l], e * 2) == hipSuccess;
-    if (train) ok &= hipMalloc(&n->h[l], e * 2) == hipSuccess;
+    if (ok) ok &= hipMemset(n->y[l], 0, e * 2) == hipSuccess;   // borders stay zero for good
+    if (train) ok &= hipMalloc(&n->h[l], act * 2) == hipSuccess;
   }
-  ok &= hipMalloc(&n->d, c * 512 * 4) == hipSuccess;
+  ok &= hipMalloc(&n->d, c * kCnnOutputs * 4) == hipSuccess;
   if (train) {
-    ok &= hipMalloc(&n->dh, c * sz3(16) * 16 * 2) == hipSuccess;
-    ok &= hipMalloc(&n->dd, c * 512 * 4) == hipSuccess;
-    ok &= hipMalloc(&n->dyA, c * sz3(16) * 16 * 4) == hipSuccess;
-    ok &= hipMalloc(&n->dyB, c * sz3(16) * 16 * 4) == hipSuccess;
-    size_t pf = (size_t)cnn_bwd_weight_blocks(count, 16) * 16 * kK;       // largest weight partials
-    pf = std::max(pf, (size_t)cnn_bwd_weight_blocks(count, 18) * 16 * kK1);
-    pf = std::max(pf, c * 433);
-    pf = std::max(pf, (size_t)((c * sz3(16) * 16 + 255) / 256 < 4096 ? (c * sz3(16) * 16 + 255) / 256 : 4096) * 16);
+    ok &= hipMalloc(&n->dh, act * 2) == hipSuccess;
+    ok &= hipMalloc(&n->dy6, act * 4) == hipSuccess;
+    ok &= hipMalloc(&n->dyA, act * 4) == hipSuccess;
+    ok &= hipMalloc(&n->dyB, act * 4) == hipSuccess;
+    const size_t pf = std::max((size_t)cnn_bwd_weight_blocks(count) * kW, (size_t)reduce_blocks(act) * kCnnChannels);
     ok &= hipMalloc(&n->part, pf * 4) == hipSuccess;
     n->part_floats = pf;
     ok &= hipMalloc(&n->lossp, kMseBlocks * 4) == hipSuccess;
@@ -116,60 +114,43 @@ int ensure_ws(pgamd_cnn* n, int count, bool train) {
     if (e_ != hipSuccess) return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e_)); \
   } while (0)
 
-// BF16 copies used by the forward and input-gradient passes.
 int refresh_copies(pgamd_cnn* n, hipStream_t s) {
-  CK(cnn_to_bf16(n->p + oW[1], n->wb, 16 * kK1, s));
-  for (int l = 2; l <= 5; l++) {
-    CK(cnn_to_bf16(n->p + oW[l], n->wb + 16 * kK1 + (l - 2) * 16 * kK, 16 * kK, s));
-    CK(cnn_transpose_w(n->p + oW[l], n->wt + (l - 2) * 16 * kK, s));
-  }
-  // w6 as BF16-rounded FP32 values, b6 as is
-  CK(cnn_round_bf16(n->p + oW6, n->w6r, 432, s));
-  CK(hipMemcpyAsync(n->w6r + 432, n->p + oB6, 4, hipMemcpyDeviceToDevice, s));
+  CK(cnn_weight_copies(n->p, n->wb, n->wt, s));
   return PGAMD_OK;
 }
-
-const uint16_t* wb_layer(const pgamd_cnn* n, int l) { return l == 1 ? n->wb : n->wb + 16 * kK1 + (l - 2) * 16 * kK; }
 
 int forward(pgamd_cnn* n, const uint16_t* x0, int count, bool keep_h, hipStream_t s) {
   const uint16_t* in = x0;
   for (int l = 1; l <= 5; l++) {
-    CK(cnn_conv_fwd(in, wb_layer(n, l), n->p + oB[l], n->y[l], keep_h ? n->h[l] : nullptr, count, kSin[l],
-                    l == 1 ? kCin1 : 16, l > 1, s));
+    CK(cnn_conv_fwd(in, n->wb + (size_t)(l - 1) * kW, n->p + ob(l), n->y[l], keep_h ? n->h[l] : nullptr, nullptr,
+                    count, l == 1 ? 0 : 1, s));
     in = n->y[l];
   }
-  CK(cnn_conv6_fwd(n->y[5], n->w6r, n->w6r + 432, n->d, count, s));
+  CK(cnn_conv_fwd(in, n->wb + (size_t)5 * kW, n->p + ob(6), nullptr, nullptr, n->d, count, 2, s));
   return PGAMD_OK;
 }
 
-__attribute__((unused)) int zero_grad(pgamd_cnn* n, hipStream_t s) {
-  CK(hipMemsetAsync(n->g, 0, kNParams * 4, s));
-  return PGAMD_OK;
-}
-
-// Forward + backward; gradients (mean-squared-error loss) accumulate into n->g, loss partials into lossp.
+// Forward + backward; gradients (mean-squared-error loss) go to n->g, loss partials to lossp.
 int forward_backward(pgamd_cnn* n, const uint16_t* x0, const float* target, int count, hipStream_t s) {
   int rc = forward(n, x0, count, true, s);
   if (rc != PGAMD_OK) return rc;
-  CK(hipMemsetAsync(n->g, 0, kNParams * 4, s));
-  CK(cnn_mse(n->d, target, n->dd, n->lossp, count * 512, kMseBlocks, s));
-  CK(cnn_conv6_bwd(n->y[5], n->w6r, n->dd, n->dyA, n->part, count, s));
-  CK(cnn_reduce_parts(n->part, count, 433, n->g + oW6, s));
-  float* dy = n->dyA;
-  float* dx = n->dyB;
-  for (int l = 5; l >= 1; l--) {
-    const int S = kSin[l], So = S - 2;
-    const size_t elems = (size_t)count * sz3(So) * 16;
-    int nb = (int)std::min<size_t>((elems + 255) / 256, 4096);
-    CK(cnn_relu_bwd(dy, n->h[l], n->dh, n->part, elems, nb, s));
-    CK(cnn_reduce_parts(n->part, nb, 16, n->g + oB[l], s));
-    const uint16_t* xin = l == 1 ? x0 : n->y[l - 1];
-    const int cin = l == 1 ? kCin1 : 16;
-    CK(cnn_conv_bwd_weight(xin, n->dh, n->part, count, S, cin, s));
-    CK(cnn_reduce_parts(n->part, cnn_bwd_weight_blocks(count, S), 16 * 27 * cin, n->g + oW[l], s));
+  const size_t act = (size_t)count * kCnnVox * kCnnChannels;
+  const int nb = reduce_blocks(act);
+  const int wblocks = cnn_bwd_weight_blocks(count);
+  CK(cnn_mse(n->d, target, n->dy6, n->lossp, count * kCnnVox, kMseBlocks, s));
+  const float* dy = n->dy6;
+  float* bufs[2] = {n->dyA, n->dyB};
+  int which = 0;
+  for (int l = 6; l >= 1; l--) {
+    CK(cnn_relu_bwd(dy, l == 6 ? nullptr : n->h[l], n->dh, n->part, act, nb, s));
+    CK(cnn_reduce_parts(n->part, nb, kCnnChannels, n->g + ob(l), s));
+    CK(cnn_conv_bwd_weight(l == 1 ? x0 : n->y[l - 1], n->dh, n->part, count, s));
+    CK(cnn_reduce_parts(n->part, wblocks, kW, n->g + ow(l), s));
     if (l > 1) {
-      CK(cnn_conv_bwd_data(n->dh, n->wt + (l - 2) * 16 * kK, l > 1 ? dy : nullptr, dx, count, S, s));
-      std::swap(dy, dx);
+      float* dx = bufs[which];
+      which ^= 1;
+      CK(cnn_conv_bwd_data(n->dh, n->wt + (size_t)(l - 1) * kW, l == 6 ? nullptr : dy, dx, count, s));
+      dy = dx;
     }
   }
   return PGAMD_OK;
@@ -177,44 +158,43 @@ int forward_backward(pgamd_cnn* n, const uint16_t* x0, const float* target, int 
 
 float read_loss(pgamd_cnn* n, int count, hipStream_t s) {
   float h[kMseBlocks];
-  if (hipMemcpyAsync(h, n->lossp, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+  if (hipMemcpyAsync(h, n->lossp, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
     return NAN;
   double t = 0.0;
   for (float x : h) t += x;
-  return (float)(t / ((double)count * 512));
+  return (float)(t / ((double)count * kCnnOutputs));
 }
 
 }  // namespace
 
 extern "C" {
 
-int pgamd_cnn_num_params(void) { return kNParams; }
+int pgamd_cnn_num_params(void) { return kCnnParams; }
 
 int pgamd_cnn_create(pgamd_context* ctx, uint64_t seed, pgamd_cnn** out) {
   if (!ctx || !out) return set_error(PGAMD_ERR_ARG, "null argument");
   *out = nullptr;
   auto* n = new pgamd_cnn();
   n->device = ctx_device(ctx);
-  if (hipSetDevice(n->device) != hipSuccess || hipMalloc(&n->p, kNParams * 4) != hipSuccess ||
-      hipMalloc(&n->g, kNParams * 4) != hipSuccess || hipMalloc(&n->m, kNParams * 4) != hipSuccess ||
-      hipMalloc(&n->v, kNParams * 4) != hipSuccess ||
-      hipMalloc(&n->wb, (16 * kK1 + 4 * 16 * kK) * 2) != hipSuccess ||
-      hipMalloc(&n->wt, 4 * 16 * kK * 2) != hipSuccess || hipMalloc(&n->w6r, 433 * 4) != hipSuccess) {
+  if (hipSetDevice(n->device) != hipSuccess || hipMalloc(&n->p, kCnnParams * 4) != hipSuccess ||
+      hipMalloc(&n->g, kCnnParams * 4) != hipSuccess || hipMalloc(&n->m, kCnnParams * 4) != hipSuccess ||
+      hipMalloc(&n->v, kCnnParams * 4) != hipSuccess || hipMalloc(&n->wb, kCnnLayers * kW * 2) != hipSuccess ||
+      hipMalloc(&n->wt, kCnnLayers * kW * 2) != hipSuccess) {
     pgamd_cnn_destroy(n);
     return set_error(PGAMD_ERR_NOMEM, "CNN parameter allocation failed");
   }
-  // He-normal initialisation; residual branches start small and the output layer at zero, so an untrained
+  // He-normal first layer, residual branches scaled by 0.1, output layer and biases zero: an untrained
   // network returns the interpolated guess unchanged.
-  std::vector<float> h(kNParams, 0.0f);
+  std::vector<float> h(kCnnParams, 0.0f);
   std::mt19937_64 rng(seed);
   std::normal_distribution<float> nd(0.0f, 1.0f);
-  for (int co = 0; co < 16; co++)
-    for (int tap = 0; tap < 27; tap++)
-      for (int ci = 0; ci < 2; ci++) h[oW[1] + co * kK1 + tap * kCin1 + ci] = nd(rng) * std::sqrt(2.0f / 54.0f);
+  const float he = std::sqrt(2.0f / kCnnK);
+  for (int i = 0; i < kW; i++) h[ow(1) + i] = nd(rng) * he;
   for (int l = 2; l <= 5; l++)
-    for (int i = 0; i < 16 * kK; i++) h[oW[l] + i] = 0.1f * nd(rng) * std::sqrt(2.0f / kK);
-  if (hipMemcpy(n->p, h.data(), kNParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(n->m, 0, kNParams * 4) != hipSuccess || hipMemset(n->v, 0, kNParams * 4) != hipSuccess) {
+    for (int i = 0; i < kW; i++) h[ow(l) + i] = 0.1f * nd(rng) * he;
+  if (hipMemcpy(n->p, h.data(), kCnnParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(n->m, 0, kCnnParams * 4) != hipSuccess || hipMemset(n->v, 0, kCnnParams * 4) != hipSuccess) {
     pgamd_cnn_destroy(n);
     return set_error(PGAMD_ERR_DEVICE, "CNN initialisation failed");
   }
@@ -238,7 +218,6 @@ int pgamd_cnn_destroy(pgamd_cnn* n) {
   hipFree(n->v);
   hipFree(n->wb);
   hipFree(n->wt);
-  hipFree(n->w6r);
   hipFree(n->x0);
   hipFree(n->scale);
   delete n;
@@ -246,19 +225,19 @@ int pgamd_cnn_destroy(pgamd_cnn* n) {
 }
 
 int pgamd_cnn_get_params(pgamd_cnn* n, float* host, int count) {
-  if (!n || !host || count != kNParams) return set_error(PGAMD_ERR_ARG, "parameter buffer must hold 31617 floats");
+  if (!n || !host || count != kCnnParams) return set_error(PGAMD_ERR_ARG, "parameter buffer must hold 41568 floats");
   hipSetDevice(n->device);
-  return hipMemcpy(host, n->p, kNParams * 4, hipMemcpyDeviceToHost) == hipSuccess
+  return hipMemcpy(host, n->p, kCnnParams * 4, hipMemcpyDeviceToHost) == hipSuccess
              ? PGAMD_OK
              : set_error(PGAMD_ERR_DEVICE, "parameter copy failed");
 }
 
 // Sets the FP32 master weights, resets the optimiser state and refreshes the BF16 copies.
 int pgamd_cnn_set_params(pgamd_cnn* n, const float* host, int count) {
-  if (!n || !host || count != kNParams) return set_error(PGAMD_ERR_ARG, "parameter buffer must hold 31617 floats");
+  if (!n || !host || count != kCnnParams) return set_error(PGAMD_ERR_ARG, "parameter buffer must hold 41568 floats");
   hipSetDevice(n->device);
-  if (hipMemcpy(n->p, host, kNParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(n->m, 0, kNParams * 4) != hipSuccess || hipMemset(n->v, 0, kNParams * 4) != hipSuccess)
+  if (hipMemcpy(n->p, host, kCnnParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(n->m, 0, kCnnParams * 4) != hipSuccess || hipMemset(n->v, 0, kCnnParams * 4) != hipSuccess)
     return set_error(PGAMD_ERR_DEVICE, "parameter copy failed");
   n->step = 0;
   return refresh_copies(n, nullptr);
@@ -272,7 +251,7 @@ int pgamd_cnn_forward(pgamd_cnn* n, const uint16_t* d_x0, int count, float* d_ou
   if (rc != PGAMD_OK) return rc;
   rc = forward(n, d_x0, count, false, s);
   if (rc != PGAMD_OK) return rc;
-  CK(hipMemcpyAsync(d_out, n->d, (size_t)count * 512 * 4, hipMemcpyDeviceToDevice, s));
+  CK(hipMemcpyAsync(d_out, n->d, (size_t)count * kCnnOutputs * 4, hipMemcpyDeviceToDevice, s));
   return PGAMD_OK;
 }
 
@@ -285,7 +264,7 @@ int pgamd_cnn_loss_grad(pgamd_cnn* n, const uint16_t* d_x0, const float* d_targe
   if (rc != PGAMD_OK) return rc;
   rc = forward_backward(n, d_x0, d_target, count, s);
   if (rc != PGAMD_OK) return rc;
-  if (d_grad) CK(hipMemcpyAsync(d_grad, n->g, kNParams * 4, hipMemcpyDeviceToDevice, s));
+  if (d_grad) CK(hipMemcpyAsync(d_grad, n->g, kCnnParams * 4, hipMemcpyDeviceToDevice, s));
   if (loss) *loss = read_loss(n, count, s);
   return PGAMD_OK;
 }
@@ -302,7 +281,7 @@ int pgamd_cnn_train_step(pgamd_cnn* n, const uint16_t* d_x0, const float* d_targ
   n->step++;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
-  CK(cnn_adam(n->p, n->g, n->m, n->v, kNParams, lr, b1, b2, eps, c1, c2, s));
+  CK(cnn_adam(n->p, n->g, n->m, n->v, kCnnParams, lr, b1, b2, eps, c1, c2, s));
   rc = refresh_copies(n, s);
   if (rc != PGAMD_OK) return rc;
   if (loss) *loss = read_loss(n, count, s);
@@ -335,7 +314,7 @@ int pgamd_cnn_warm_start(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho, c
     n->x0 = nullptr;
     n->scale = nullptr;
     n->icap = 0;
-    if (hipMalloc(&n->x0, (size_t)npatch * sz3(18) * 8 * 2) != hipSuccess ||
+    if (hipMalloc(&n->x0, (size_t)npatch * kCnnInput * 2) != hipSuccess ||
         hipMalloc(&n->scale, (size_t)npatch * 4) != hipSuccess)
       return set_error(PGAMD_ERR_NOMEM, "warm-start workspace allocation failed");
     n->icap = npatch;

@@ -22,7 +22,7 @@ struct SolveParams {
   int check_every;    // relative-residual cadence
   int init_mode;      // 0: interpolated cube; 1: zero interior; 2: caller guess
   int int_scheme;     // 4 CQUAD, 5 QUAD
-  int gra_ghost;      // GRA_GHOST_SIZE of the output window (1 or 2)
+  int gra_ghost;      // GRA_GHOST_SIZE of the output window (1..gs; > 2 only without gravity)
   // multigrid
   float poi_coeff;
   float dh;

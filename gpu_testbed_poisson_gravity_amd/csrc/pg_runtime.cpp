@@ -126,8 +126,8 @@ int check_mixed_args(pgamd_context* c, pgamd_plan* pl, const float* rho, const f
     for (int g = 1; g < 2; g++)
       if (pl->has_gs[g]) return set_error(PGAMD_ERR_ARG, "the 5-point gradient needs every ghost size >= 2");
   }
-  if (o.gra_ghost_size > 1 && pl->has_gs[1])
-    return set_error(PGAMD_ERR_ARG, "GRA_GHOST_SIZE exceeds the ghost size of some patches");
+  for (int g = 1; g < o.gra_ghost_size && g <= kMaxGhost; g++)
+    if (pl->has_gs[g]) return set_error(PGAMD_ERR_ARG, "GRA_GHOST_SIZE exceeds the ghost size of some patches");
   if ((a->poisson && (!rho || !cpot)) || !pot_out || (a->graacc && !flu))
     return set_error(PGAMD_ERR_ARG, "null device array");
   if (a->poisson && o.init_mode == PGAMD_INIT_GUESS && !guess)

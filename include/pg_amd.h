@@ -61,7 +61,8 @@ typedef struct pgamd_context pgamd_context;
 /* Options the reference fixes with compile-time macros (Makefile:10-53, Header/CUPOT.h). */
 typedef struct {
   int pot_ghost_size;   /* POT_GHOST_SIZE, 1..5                                   */
-  int gra_ghost_size;   /* GRA_GHOST_SIZE, 1..2, <= pot_ghost_size                 */
+  int gra_ghost_size;   /* GRA_GHOST_SIZE, 1..2, <= pot_ghost_size; a Poisson-only call (graacc = 0)
+                           may use up to pot_ghost_size: pot_out is then the whole solve cube */
   int pot_scheme;       /* PGAMD_SOR or PGAMD_MG                                   */
   int init_mode;        /* PGAMD_INIT_*                                            */
   int term_mode;        /* PGAMD_TERM_*                                            */
@@ -178,24 +179,27 @@ int pgamd_graph_launch(pgamd_graph* graph, void* stream);
 int pgamd_graph_destroy(pgamd_graph* graph);
 
 /* ---- learned warm start (no reference counterpart) ----------------------------------------------------
- * A 3D CNN (BF16 weights, FP32 accumulation, hand-written MFMA convolutions) predicts the patch-proper 8^3
- * part of the converged potential from an 18^3 input with 8 BF16 channels per voxel ([n][18][18][18][8]):
- * channel 0 = interpolated coarse potential / s, channel 1 = Poi_Coeff*dh^2*rho / s on the solve interior,
- * channels 2..7 = 0, where s = max|interpolated potential| of the patch (smaller cubes are centred with
- * zeros around).  The prediction d ([n][8^3], FP32) gives the guess interp + s*d on the patch proper and
- * the interpolated values elsewhere; solve with init_mode PGAMD_INIT_GUESS and, for a safe fallback,
- * guess_fallback_tol > 0.  Parameters: 31617 FP32 values (layer order W1 b1 W2 b2 ... W5 b5 w6 b6). */
+ * A 3D CNN (BF16 weights, FP32 accumulation, hand-written MFMA convolutions) predicts the correction from
+ * the interpolated coarse potential to the converged potential on the whole solve interior.  Frame: the
+ * 18^3 solve cube of ghost size 5 (smaller cubes centred, zeros around) with q0 = interp/s and
+ * q1 = Poi_Coeff*dh^2*rho/s (interior only), s = max|interp| of the patch.  Input x0 [n][10^3][16] BF16:
+ * space-to-depth of the zero-extended 20^3 frame (fine index f = 2c-1+d per axis, channel
+ * ((dz*2+dy)*2+dx)*2 + q).  Six 3^3 convolutions with 16 channels on the 8^3 coarse grid (the first
+ * valid on the 10^3 input, the next four residual blocks y += relu(conv(y)+b), the last linear with 8
+ * outputs) give d [n][8^3][8] FP32; the guess at interior frame index 2o+p is interp + s*d[o][p].  Solve
+ * with init_mode PGAMD_INIT_GUESS and, for a safe fallback, guess_fallback_tol > 0.
+ * Parameters: 41568 FP32 values, per layer W [16][432] (W[co][tap*16+ci], tap = dz*9+dy*3+dx) then b [16]. */
 typedef struct pgamd_cnn pgamd_cnn;
 int pgamd_cnn_num_params(void);
 int pgamd_cnn_create(pgamd_context* ctx, uint64_t seed, pgamd_cnn** net);
 int pgamd_cnn_destroy(pgamd_cnn* net);
 int pgamd_cnn_get_params(pgamd_cnn* net, float* host_params, int count);
 int pgamd_cnn_set_params(pgamd_cnn* net, const float* host_params, int count);
-/* Network input of a batch of same-ghost-size patches (d_x0: [n][18^3][8] BF16 bits, d_scale: [n]). */
+/* Network input of a batch of same-ghost-size patches (d_x0: [n][10^3][16] BF16 bits, d_scale: [n]). */
 int pgamd_cnn_prepare(pgamd_context* ctx, const float* d_rho, const float* d_pot_in, int npatch, int ghost_size,
                       const pgamd_solver_args* args, uint16_t* d_x0, float* d_scale, void* stream);
 int pgamd_cnn_forward(pgamd_cnn* net, const uint16_t* d_x0, int count, float* d_out, void* stream);
-/* Mean-squared-error loss against d_target ([n][8^3]) and its gradient (FP32, 31617 values, may be NULL);
+/* Mean-squared-error loss against d_target ([n][8^3][8]) and its gradient (FP32, 41568 values, may be NULL);
  * *loss (may be NULL) is read back synchronously. */
 int pgamd_cnn_loss_grad(pgamd_cnn* net, const uint16_t* d_x0, const float* d_target, int count, float* loss,
                         float* d_grad, void* stream);

@@ -1,27 +1,34 @@
-"""CPU oracle of the warm-start CNN (forward, mean-squared-error loss and backward), in numpy FP64.
+"""CPU oracle of the warm-start CNN (input packing, forward, mean-squared-error loss and backward), numpy.
 
-TEST INFRASTRUCTURE ONLY: used by tests/ to check the HIP forward and backward kernels.  The network has
-no reference counterpart; its definition is the one stated in include/pg_amd.h (pgamd_cnn_*):
+TEST INFRASTRUCTURE ONLY: used by tests/ to check the HIP kernels.  The network has no reference
+counterpart; its definition is the one stated in include/pg_amd.h (pgamd_cnn_*):
 
-    x0 [n,18,18,18,8] -> y1 = relu(conv(x0)+b1)                       (valid 3^3 conv, 18 -> 16)
-    y_l = crop(y_{l-1}) + relu(conv(y_{l-1}) + b_l),  l = 2..5        (16 -> 8)
-    d = conv_same(y5) + b6                                            (8^3, 1 channel)
+    fields on the 18^3 frame (the solve cube of ghost size 5; smaller cubes centred, zeros around):
+      q0 = interp / s,  q1 = Poi_Coeff*dh^2*rho / s (solve interior only),  s = max |interp|
+    x0 [n,10,10,10,16]: space-to-depth of the 20^3 zero-extended frame, fine index f = 2c - 1 + d
+      (c = 0..9, d = 0..1), channel ((dz*2+dy)*2+dx)*2 + q
+    y1 = relu(conv(x0) + b1)                                  (valid 3^3 conv, 10^3 -> 8^3)
+    y_l = y_{l-1} + relu(conv(pad(y_{l-1})) + b_l), l = 2..5   ('same', zero padding, 8^3)
+    D = conv(pad(y5)) + b6, 8 channels                         (linear)
+    the correction on the 16^3 interior frame (fine f = 1..16): corr[2o+p] = D[o][p],
+      p = (pz*2+py)*2+px, o = (oz*8+oy)*8+ox;  guess = interp + s*corr
 
-Parameters are the flat vector of pgamd_cnn_get_params (W1 b1 W2 b2 W3 b3 W4 b4 W5 b5 w6 b6) with
-W_l[co][tap*CIN + ci], tap = dz*9 + dy*3 + dx.  With bf16=True the activations, the BF16 weight copies
-and the ReLU gradients are rounded to BF16 at the same points as the GPU kernels (to compare closely);
-with bf16=False everything is FP64 (used for the finite-difference pin of the backward pass).
+Parameters (pgamd_cnn_get_params order): for l = 1..6, W_l [16][432] then b_l [16]; W_l[co][tap*16+ci],
+tap = dz*9 + dy*3 + dx; rows 8..15 of W6 and b6[8..15] are unused (zero).  With bf16=True, weights,
+stored activations, pre-activations and the gradients fed to the matrix cores are rounded to BF16 at the
+same points as the kernels; with bf16=False everything is FP64 (used to pin the backward pass against
+finite differences).
 """
 from __future__ import annotations
 
 import numpy as np
 from numpy.lib.stride_tricks import sliding_window_view
 
-NPARAMS = 31617
-OW = {1: 0, 2: 3472, 3: 10400, 4: 17328, 5: 24256}
-OB = {1: 3456, 2: 10384, 3: 17312, 4: 24240, 5: 31168}
-OW6, OB6 = 31184, 31616
-SIN = {1: 18, 2: 16, 3: 14, 4: 12, 5: 10}
+LAYER = 16 * 432 + 16
+NPARAMS = 6 * LAYER
+OW = {l: (l - 1) * LAYER for l in range(1, 7)}
+OB = {l: OW[l] + 16 * 432 for l in range(1, 7)}
+NOUT = 8
 
 
 def bf16_round(a):
@@ -32,21 +39,60 @@ def bf16_round(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def bf16_bits_to_float(bits):
-    return (np.asarray(bits, np.uint16).astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+def bf16_bits(a):
+    """BF16 bit patterns (uint16) of the rounding of a."""
+    return (bf16_round(a).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
 
 
 def unpack(p):
     p = np.asarray(p, np.float64)
-    W = {1: p[OW[1]:OW[1] + 16 * 216].reshape(16, 216)}
-    for l in range(2, 6):
-        W[l] = p[OW[l]:OW[l] + 16 * 432].reshape(16, 432)
-    b = {l: p[OB[l]:OB[l] + 16] for l in range(1, 6)}
-    return W, b, p[OW6:OW6 + 432], p[OB6]
+    W = {l: p[OW[l]:OW[l] + 16 * 432].reshape(16, 432) for l in range(1, 7)}
+    b = {l: p[OB[l]:OB[l] + 16] for l in range(1, 7)}
+    return W, b
 
+
+# ---- input packing and output unpacking --------------------------------------------------------------
+
+def frame18(cube, rho, gs, cst, scale):
+    """(q0, q1) on the 18^3 frame for one patch: cube [N^3] interpolated potential, rho [R^3]."""
+    N, R, off = 8 + 2 * gs, 6 + 2 * gs, 5 - gs
+    q0 = np.zeros((18, 18, 18), np.float32)
+    q1 = np.zeros((18, 18, 18), np.float32)
+    inv = np.float32(1.0) / np.float32(scale)
+    q0[off:off + N, off:off + N, off:off + N] = np.asarray(cube, np.float32) * inv
+    q1[off + 1:off + 1 + R, off + 1:off + 1 + R, off + 1:off + 1 + R] = \
+        (np.float32(cst) * np.asarray(rho, np.float32)) * inv
+    return q0, q1
+
+
+def space_to_depth(q0, q1):
+    """[n,18,18,18] x 2 -> [n,10,10,10,16] (fine index f = 2c - 1 + d)."""
+    n = q0.shape[0]
+    ext = np.zeros((n, 20, 20, 20, 2), np.float64)
+    ext[:, 1:19, 1:19, 1:19, 0] = q0
+    ext[:, 1:19, 1:19, 1:19, 1] = q1
+    x = ext.reshape(n, 10, 2, 10, 2, 10, 2, 2).transpose(0, 1, 3, 5, 2, 4, 6, 7)
+    return x.reshape(n, 10, 10, 10, 16)
+
+
+def depth_to_space(D):
+    """[n,512,8] -> correction on the 16^3 interior frame [n,16,16,16]."""
+    n = D.shape[0]
+    d = np.asarray(D).reshape(n, 8, 8, 8, 2, 2, 2).transpose(0, 1, 4, 2, 5, 3, 6)
+    return d.reshape(n, 16, 16, 16)
+
+
+def interior_to_depth(corr):
+    """Adjoint layout of depth_to_space: [n,16,16,16] -> [n,512,8]."""
+    n = corr.shape[0]
+    d = np.asarray(corr).reshape(n, 8, 2, 8, 2, 8, 2).transpose(0, 1, 3, 5, 2, 4, 6)
+    return d.reshape(n, 512, 8)
+
+
+# ---- network ------------------------------------------------------------------------------------------
 
 def _windows(x):
-    """[n,S,S,S,c] -> [n,So,So,So,27*c] with K ordered tap-major (tap*c + ci)."""
+    """[n,S,S,S,c] -> [n,S-2,S-2,S-2,27*c] with K ordered tap-major (tap*c + ci)."""
     n, S, _, _, c = x.shape
     w = sliding_window_view(x, (3, 3, 3), axis=(1, 2, 3))          # [n,So,So,So,c,3,3,3]
     w = np.moveaxis(w, 4, 7)                                         # [n,So,So,So,3,3,3,c]
@@ -54,7 +100,7 @@ def _windows(x):
 
 
 def _scatter_windows(g, S, c):
-    """Adjoint of _windows: [n,So,So,So,27*c] -> [n,S,S,S,c]."""
+    """Adjoint of _windows: [n,S-2,S-2,S-2,27*c] -> [n,S,S,S,c]."""
     n, So = g.shape[0], S - 2
     g = g.reshape(n, So, So, So, 3, 3, 3, c)
     out = np.zeros((n, S, S, S, c))
@@ -65,58 +111,62 @@ def _scatter_windows(g, S, c):
     return out
 
 
+def _pad(y):
+    n = y.shape[0]
+    out = np.zeros((n, 10, 10, 10, 16))
+    out[:, 1:9, 1:9, 1:9, :] = y
+    return out
+
+
 def forward(p, x0, bf16=True):
-    """Returns (d [n,512], cache)."""
-    W, b, w6, b6 = unpack(p)
+    """x0 [n,10,10,10,16] -> (D [n,512,8], cache)."""
+    W, b = unpack(p)
     r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
     Wq = {l: r(W[l]) for l in W}
-    w6q = r(w6)
     x = np.asarray(x0, np.float64)
-    cache = {"x": {1: x}, "h": {}, "win": {}}
-    for l in range(1, 6):
-        win = _windows(x)
+    cache = {"in": {}, "h": {}, "win": {}, "Wq": Wq}
+    y = None
+    for l in range(1, 7):
+        xin = x if l == 1 else _pad(y)
+        win = _windows(xin)                                          # [n,8,8,8,432]
         h = win @ Wq[l].T + b[l]
-        y = np.maximum(h, 0.0)
+        cache["win"][l] = win
+        if l == 6:
+            D = h[..., :NOUT]
+            break
+        yn = np.maximum(h, 0.0)
         if l > 1:
-            y = y + x[:, 1:-1, 1:-1, 1:-1, :]
+            yn = yn + y
         if bf16:
-            h = r(h)
-            y = r(y)
-        cache["win"][l], cache["h"][l] = win, h
-        x = y
-        cache["x"][l + 1] = x
-    xp = np.zeros((x.shape[0], 10, 10, 10, 16))
-    xp[:, 1:9, 1:9, 1:9, :] = x
-    win6 = _windows(xp)                                               # [n,8,8,8,432]
-    d = win6 @ w6q + b6
-    cache["win6"] = win6
-    cache["Wq"], cache["w6q"] = Wq, w6q
-    return d.reshape(-1, 512), cache
+            h, yn = r(h), r(yn)
+        cache["h"][l] = h
+        y = yn
+    return D.reshape(-1, 512, NOUT), cache
 
 
 def loss_grad(p, x0, target, bf16=True):
-    """Mean-squared-error loss and its gradient with respect to the parameter vector."""
-    d, c = forward(p, x0, bf16)
-    n = d.shape[0]
-    t = np.asarray(target, np.float64).reshape(n, 512)
-    e = d - t
+    """Mean-squared-error loss over all n*512*8 outputs and its gradient (parameter vector order)."""
+    D, c = forward(p, x0, bf16)
+    n = D.shape[0]
+    t = np.asarray(target, np.float64).reshape(n, 512, NOUT)
+    e = D - t
     loss = float(np.mean(e * e))
-    dd = (2.0 * e / e.size).reshape(n, 8, 8, 8)
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
+    dD = 2.0 * e / e.size
+    dh = np.zeros((n, 8, 8, 8, 16))
+    dh[..., :NOUT] = r(dD).reshape(n, 8, 8, 8, NOUT)
     g = np.zeros(NPARAMS)
-    g[OW6:OW6 + 432] = np.einsum("nzyx,nzyxk->k", dd, c["win6"])
-    g[OB6] = dd.sum()
-    dwin6 = dd[..., None] * c["w6q"]                                  # [n,8,8,8,432]
-    dy = _scatter_windows(dwin6, 10, 16)[:, 1:9, 1:9, 1:9, :]
-    r = bf16_round if bf16 else (lambda a: a)
-    for l in range(5, 0, -1):
-        dh = np.where(c["h"][l] > 0, dy, 0.0)
-        if bf16:
-            dh = r(dh)
+    dy = None
+    for l in range(6, 0, -1):
+        if l < 6:
+            dh = np.where(c["h"][l] > 0, dy, 0.0)
+            if bf16:
+                dh = r(dh)
         g[OB[l]:OB[l] + 16] = dh.sum(axis=(0, 1, 2, 3))
-        cin = 8 if l == 1 else 16
-        g[OW[l]:OW[l] + 16 * 27 * cin] = np.einsum("nzyxc,nzyxk->ck", dh, c["win"][l]).ravel()
+        g[OW[l]:OW[l] + 16 * 432] = np.einsum("nzyxc,nzyxk->ck", dh, c["win"][l]).ravel()
         if l > 1:
-            dx = _scatter_windows(dh @ c["Wq"][l], SIN[l], 16)
-            dx[:, 1:-1, 1:-1, 1:-1, :] += dy
+            dx = _scatter_windows(dh @ c["Wq"][l], 10, 16)[:, 1:9, 1:9, 1:9, :]
+            if l < 6:
+                dx = dx + dy
             dy = dx
     return loss, g

@@ -1,4 +1,5 @@
-"""Pins on the CNN oracle (CPU): the analytic backward pass against central finite differences in FP64."""
+"""Pins on the CNN oracle (CPU): the analytic backward pass against central finite differences in FP64,
+the input/output packing against direct index arithmetic, and BF16 rounding against hand-worked values."""
 import numpy as np
 import pytest
 
@@ -8,16 +9,20 @@ from oracle import cnn_oracle as K
 def _random_problem(seed, n=1):
     rng = np.random.default_rng(seed)
     p = rng.standard_normal(K.NPARAMS) * 0.05
-    x0 = np.zeros((n, 18, 18, 18, 8))
-    x0[..., :2] = rng.standard_normal((n, 18, 18, 18, 2))
-    t = rng.standard_normal((n, 512)) * 0.1
+    for l in range(1, 7):
+        p[K.OB[l]:K.OB[l] + 16] = rng.uniform(0.0, 0.1, 16)   # keep most ReLUs open
+    W6 = p[K.OW[6]:K.OW[6] + 16 * 432].reshape(16, 432)
+    W6[K.NOUT:] = 0.0
+    p[K.OB[6] + K.NOUT:K.OB[6] + 16] = 0.0
+    x0 = rng.standard_normal((n, 10, 10, 10, 16))
+    t = rng.standard_normal((n, 512, K.NOUT)) * 0.1
     return p, x0, t
 
 
 def test_parameter_layout():
-    W, b, w6, b6 = K.unpack(np.arange(K.NPARAMS, dtype=np.float64))
-    assert W[1].shape == (16, 216) and W[5].shape == (16, 432) and w6.shape == (432,)
-    assert b[5][-1] + 1 == K.OW6 and b6 == K.NPARAMS - 1
+    W, b = K.unpack(np.arange(K.NPARAMS, dtype=np.float64))
+    assert all(W[l].shape == (16, 432) for l in range(1, 7))
+    assert b[1][0] == 16 * 432 and W[2][0, 0] == K.LAYER and b[6][-1] == K.NPARAMS - 1
 
 
 @pytest.mark.parametrize("seed", [0, 1])
@@ -25,9 +30,9 @@ def test_backward_matches_finite_differences(seed):
     p, x0, t = _random_problem(seed)
     _, g = K.loss_grad(p, x0, t, bf16=False)
     rng = np.random.default_rng(100 + seed)
-    # probe weights and biases of every layer (the padded input channels 2..7 of W1 have zero gradient)
-    probes = [K.OW[1] + 5 * 216 + 3 * 8 + 1, K.OB[1] + 2, K.OW[3] + 7 * 432 + 200, K.OB[4] + 9,
-              K.OW[5] + rng.integers(0, 16 * 432), K.OW6 + 17, K.OB6, K.OW[2] + 431]
+    probes = [K.OW[1] + 5 * 432 + 3 * 16 + 1, K.OB[1] + 2, K.OW[3] + 7 * 432 + 200, K.OB[4] + 9,
+              K.OW[5] + int(rng.integers(0, 16 * 432)), K.OW[6] + 3 * 432 + 17, K.OB[6] + 5, K.OW[2] + 431,
+              K.OW[4] + 15 * 432 + 430]
     eps = 1e-6
     for i in probes:
         e = np.zeros(K.NPARAMS)
@@ -36,17 +41,51 @@ def test_backward_matches_finite_differences(seed):
         lm, _ = K.loss_grad(p - e, x0, t, bf16=False)
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
-    assert np.all(g[K.OW[1]:K.OW[1] + 16 * 216].reshape(16, 27, 8)[:, :, 2:] == 0)
+    # the unused output channels carry no gradient
+    assert np.all(g[K.OW[6] + K.NOUT * 432:K.OB[6]] == 0) and np.all(g[K.OB[6] + K.NOUT:K.OB[6] + 16] == 0)
 
 
-def test_forward_shapes_and_identity_at_zero_output_layer():
+def test_zero_output_layer_gives_zero_correction():
     p, x0, _ = _random_problem(3, n=2)
-    p[K.OW6:K.OB6 + 1] = 0.0
+    p[K.OW[6]:] = 0.0
     d, _ = K.forward(p, x0, bf16=True)
-    assert d.shape == (2, 512) and np.all(d == 0)
+    assert d.shape == (2, 512, K.NOUT) and np.all(d == 0)
+
+
+def test_packing_index_arithmetic():
+    rng = np.random.default_rng(4)
+    q0, q1 = rng.standard_normal((2, 2, 18, 18, 18))
+    x = K.space_to_depth(q0, q1)
+    # fine index f = 2c - 1 + d; channel ((dz*2+dy)*2+dx)*2 + q
+    for _ in range(50):
+        c = rng.integers(0, 10, 3)
+        d = rng.integers(0, 2, 3)
+        q = int(rng.integers(0, 2))
+        f = 2 * c - 1 + d
+        ch = ((d[0] * 2 + d[1]) * 2 + d[2]) * 2 + q
+        want = 0.0 if np.any(f < 0) or np.any(f > 17) else (q0, q1)[q][1, f[0], f[1], f[2]]
+        assert x[1, c[0], c[1], c[2], ch] == want
+    corr = rng.standard_normal((2, 16, 16, 16))
+    D = K.interior_to_depth(corr)
+    assert np.array_equal(K.depth_to_space(D), corr)
+    # interior frame index i = 2o + p, the cell that x0 holds at coarse o+1, sub-cell p
+    o, pp = np.array([3, 0, 7]), np.array([1, 0, 1])
+    i = 2 * o + pp
+    assert D[0, (o[0] * 8 + o[1]) * 8 + o[2], (pp[0] * 2 + pp[1]) * 2 + pp[2]] == corr[0, i[0], i[1], i[2]]
+    f = i + 1
+    assert np.array_equal((f + 1) // 2, o + 1) and np.array_equal((f + 1) % 2, pp)
+
+
+def test_frame_centres_smaller_cubes():
+    cube = np.arange(10 ** 3, dtype=np.float32).reshape(10, 10, 10) + 1
+    rho = np.ones((8, 8, 8), np.float32)
+    q0, q1 = K.frame18(cube, rho, gs=1, cst=2.0, scale=4.0)
+    assert q0[4, 4, 4] == cube[0, 0, 0] / 4 and q0[3].max() == 0 and q0[14].max() == 0
+    assert q1[4].max() == 0 and q1[5, 5, 5] == 0.5 and q1[13, 12, 12] == 0
 
 
 def test_bf16_round_ties_to_even():
     v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.0e38], np.float32)
     r = K.bf16_round(v)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == -2.5
+    assert K.bf16_bits(np.float32(1.0)) == 0x3F80 and K.bf16_bits(np.float32(-2.0)) == 0xC000

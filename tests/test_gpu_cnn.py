@@ -47,19 +47,18 @@ def _bf16_bits(x):
 def _random_params(seed):
     rng = np.random.default_rng(seed)
     p = np.zeros(CO.NPARAMS)
-    p[CO.OW[1]:CO.OW[1] + 16 * 216] = rng.normal(0, np.sqrt(2 / 216), 16 * 216)
-    for l in range(2, 6):
-        p[CO.OW[l]:CO.OW[l] + 16 * 432] = rng.normal(0, np.sqrt(2 / 432), 16 * 432)
-    for l in range(1, 6):
+    for l in range(1, 7):
+        p[CO.OW[l]:CO.OW[l] + 16 * 432] = rng.normal(0, np.sqrt(2 / 432) * (1.0 if l in (1, 6) else 0.5), 16 * 432)
         p[CO.OB[l]:CO.OB[l] + 16] = rng.normal(0, 0.1, 16)
-    p[CO.OW6:CO.OW6 + 432] = rng.normal(0, np.sqrt(1 / 432), 432)
-    p[CO.OB6] = 0.05
+    p[CO.OW[6] + CO.NOUT * 432:CO.OB[6]] = 0.0          # unused output rows
+    p[CO.OB[6] + CO.NOUT:CO.OB[6] + 16] = 0.0
     return p.astype(np.float32)
 
 
 def _random_x0(n, seed):
     rng = np.random.default_rng(seed)
-    return _bf16_bits(rng.uniform(-1, 1, (n, 18, 18, 18, 8)).astype(np.float32))
+    x = rng.uniform(-1, 1, (n, 10, 10, 10, 16)).astype(np.float32)
+    return CO.bf16_bits(x), CO.bf16_round(x)
 
 
 def _rel(a, b):
@@ -79,9 +78,9 @@ def test_num_params_and_param_roundtrip(ctx):
     try:
         p0 = net.get_params()
         assert p0.shape == (CO.NPARAMS,)
-        W, b, w6, b6 = CO.unpack(p0)
-        assert np.all(w6 == 0) and b6 == 0 and all(np.all(b[l] == 0) for l in b)   # untrained: d = 0
-        assert np.all(W[1].reshape(16, 27, 8)[:, :, 2:] == 0)                      # unused input channels
+        W, b = CO.unpack(p0)
+        assert np.all(W[6] == 0) and all(np.all(b[l] == 0) for l in b)   # untrained: d = 0
+        assert np.all(W[1] != 0)
         p1 = _random_params(1)
         net.set_params(p1)
         assert np.array_equal(net.get_params(), p1)
@@ -100,7 +99,7 @@ def test_forward_matches_oracle(ctx, n):
     try:
         net.set_params(p)
         d_x0 = _dev(bits.view(np.int16))
-        d_out = torch.empty((n, 512), device="cuda")
+        d_out = torch.empty((n, 512, 8), device="cuda")
         print(f"[case] forward n={n}", flush=True)
         net.forward(d_x0, n, d_out)
         torch.cuda.synchronize()
@@ -120,7 +119,7 @@ def test_loss_and_gradient_match_oracle(ctx):
     n = 2
     p = _random_params(7)
     bits, x0 = _random_x0(n, seed=8)
-    tgt = np.random.default_rng(9).normal(0, 0.5, (n, 512)).astype(np.float32)
+    tgt = np.random.default_rng(9).normal(0, 0.5, (n, 512, 8)).astype(np.float32)
     net = pg.CNN(ctx)
     try:
         net.set_params(p)
@@ -133,13 +132,13 @@ def test_loss_and_gradient_match_oracle(ctx):
         net.close()
     loss_o, g_o = CO.loss_grad(p, x0, tgt, bf16=True)
     assert abs(loss - loss_o) <= 1e-4 * loss_o
-    blocks = [(f"W{l}", CO.OW[l], 16 * (216 if l == 1 else 432)) for l in range(1, 6)]
-    blocks += [(f"b{l}", CO.OB[l], 16) for l in range(1, 6)] + [("w6", CO.OW6, 432), ("b6", CO.OB6, 1)]
+    blocks = [(f"W{l}", CO.OW[l], 16 * 432) for l in range(1, 7)] + [(f"b{l}", CO.OB[l], 16) for l in range(1, 7)]
     errs = {}
     for name, o, ln in blocks:
         errs[name] = _rel(g[o:o + ln], g_o[o:o + ln])
         print(f"  {name}: rel {errs[name]:.2e}", flush=True)
-    assert np.all(np.any(g[CO.OW[1]:CO.OW[1] + 16 * 216].reshape(16, 27, 8) != 0, axis=(0, 1)))  # 8 live inputs
+    assert np.all(np.any(g[CO.OW[1]:CO.OW[1] + 16 * 432].reshape(16, 27, 16) != 0, axis=(0, 1)))  # live inputs
+    assert np.all(g[CO.OW[6] + CO.NOUT * 432:CO.OB[6]] == 0)                                  # unused outputs
     for name, e in errs.items():
         assert e < TOL_GRAD, (name, e)
 
@@ -149,7 +148,7 @@ def test_gradient_is_bitwise_reproducible(ctx):
     n = 4
     p = _random_params(3)
     bits, _ = _random_x0(n, seed=4)
-    tgt = np.random.default_rng(5).normal(0, 0.5, (n, 512)).astype(np.float32)
+    tgt = np.random.default_rng(5).normal(0, 0.5, (n, 512, 8)).astype(np.float32)
     net = pg.CNN(ctx)
     try:
         net.set_params(p)
@@ -168,8 +167,9 @@ def test_training_reduces_loss(ctx):
     torch = _torch()
     n = 8
     bits, x0 = _random_x0(n, seed=11)
-    # a learnable target: a fixed smooth function of the input's first channel on the core
-    tgt = (0.3 * x0[:, 5:13, 5:13, 5:13, 0] - 0.2 * x0[:, 5:13, 5:13, 5:13, 1]).reshape(n, 512).astype(np.float32)
+    # a learnable target: a fixed linear function of the two fields at the same fine cell
+    core = x0[:, 1:9, 1:9, 1:9, :].reshape(n, 512, 8, 2)
+    tgt = (0.3 * core[..., 0] - 0.2 * core[..., 1]).astype(np.float32)
     net = pg.CNN(ctx, seed=1)
     try:
         d_x0, d_t = _dev(bits.view(np.int16)), _dev(tgt)
@@ -198,7 +198,7 @@ def test_prepare_matches_oracle_interp(ctx, gs, ints):
     n = 5
     rho, cpot, _, _ = I.plummer(n, gs, seed=gs)
     args = pg.solver_args(dh=0.5, poi_coeff=POI, int_scheme=ints)
-    d_x0 = torch.empty((n, 18 ** 3 * 8), dtype=torch.int16, device="cuda")
+    d_x0 = torch.empty((n, pg.CNN.INPUT), dtype=torch.int16, device="cuda")
     d_s = torch.empty(n, device="cuda")
     print(f"[case] prepare gs={gs} ints={ints}", flush=True)
     net = pg.CNN(ctx)
@@ -207,21 +207,14 @@ def test_prepare_matches_oracle_interp(ctx, gs, ints):
         torch.cuda.synchronize()
     finally:
         net.close()
-    x = d_x0.cpu().numpy().view(np.uint16).reshape(n, 18, 18, 18, 8)
+    x = d_x0.cpu().numpy().view(np.uint16).reshape(n, 10, 10, 10, 16)
     s_got = d_s.cpu().numpy()
     cube, s = _scale_and_interp(cpot, gs, ints)
     assert np.array_equal(s_got, s)
-    g = pg.geometry(gs)
-    N, R, off = g["N"], g["R"], 5 - gs
-    inv = (np.float32(1.0) / s).astype(np.float32)[:, None, None, None]
-    c0 = np.zeros((n, 18, 18, 18), np.float32)
-    c0[:, off:off + N, off:off + N, off:off + N] = cube * inv
     cst = np.float32(np.float32(POI) * np.float32(0.5)) * np.float32(0.5)
-    c1 = np.zeros((n, 18, 18, 18), np.float32)
-    c1[:, off + 1:off + 1 + R, off + 1:off + 1 + R, off + 1:off + 1 + R] = (cst * rho.astype(np.float32)) * inv
-    assert np.array_equal(x[..., 0], _bf16_bits(c0)[0])
-    assert np.array_equal(x[..., 1], _bf16_bits(c1)[0])
-    assert np.all(x[..., 2:] == 0)
+    q = [CO.frame18(cube[k], rho[k], gs, cst, s[k]) for k in range(n)]
+    want = CO.space_to_depth(np.stack([a for a, _ in q]), np.stack([b for _, b in q]))
+    assert np.array_equal(x, CO.bf16_bits(want))
 
 
 def _solve_dev(ctx, rho, cpot, flu, args, guess=None):
@@ -267,18 +260,17 @@ def test_warm_start_guess_assembly(gs):
             net.set_params(_random_params(5))
             print(f"[case] warm start random gs={gs}", flush=True)
             net.warm_start(d_rho, d_cpot, n, args, d_guess)
-            d_x0 = torch.empty((n, 18 ** 3 * 8), dtype=torch.int16, device="cuda")
+            d_x0 = torch.empty((n, pg.CNN.INPUT), dtype=torch.int16, device="cuda")
             d_s = torch.empty(n, device="cuda")
-            d_d = torch.empty((n, 512), device="cuda")
+            d_d = torch.empty((n, 512, 8), device="cuda")
             net.prepare(d_rho, d_cpot, n, gs, args, d_x0, d_s)
             net.forward(d_x0, n, d_d)
             torch.cuda.synchronize()
         finally:
             net.close()
-    want = interior.copy()
-    core = slice(gs - 1, gs + 7)
-    dd = d_d.cpu().numpy().reshape(n, 8, 8, 8)
-    want[:, core, core, core] = want[:, core, core, core] + s[:, None, None, None] * dd
+    off = 5 - gs
+    corr = CO.depth_to_space(d_d.cpu().numpy())[:, off:off + R, off:off + R, off:off + R].astype(np.float32)
+    want = interior + s[:, None, None, None] * corr
     assert np.array_equal(d_guess.cpu().numpy(), want)
 
 

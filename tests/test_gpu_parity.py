@@ -227,3 +227,29 @@ def test_repeat_is_bitwise_deterministic():
                       sor_min_iter=10, sor_max_iter=1000, poi_coeff=POI)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("scheme", [pg.SOR, pg.MG])
+def test_full_cube_output_window(scheme):
+    """A Poisson-only call may widen the output window up to the whole solve cube (GRA_GHOST_SIZE = ghost
+    size): its centre equals the usual window bit for bit and its outer layer is the interpolated
+    Dirichlet boundary (oracle interpolation).  With the gravity update, the window is limited to 2."""
+    gs, n = 4, 12
+    rho, cpot, flu = _mixed_inputs(n, gs, seed=7)
+    kw = dict(gs=gs, scheme=scheme, sor_min_iter=30, sor_max_iter=30, sor_omega=OMEGA[3], mg_max_iter=4,
+              mg_tol=1e-12, poi_coeff=POI, graacc=False)
+    print(f"[case] full-cube window scheme={scheme}", flush=True)
+    full = pg.solve_host(rho, cpot, None, gg=gs, **kw)[0]
+    win = pg.solve_host(rho, cpot, None, gg=1, **kw)[0]
+    N = 8 + 2 * gs
+    assert full.shape == (n, N, N, N)
+    o = gs - 1
+    assert np.array_equal(full[:, o:o + 10, o:o + 10, o:o + 10], win)
+    cube = O.interp(cpot, gs, pg.INT_CQUAD, dtype=np.float32)
+    face = np.ones((N, N, N), bool)
+    face[1:-1, 1:-1, 1:-1] = False
+    assert np.array_equal(full[:, face], cube[:, face])
+    with pg.Context(0) as ctx:
+        ctx.set_options(pot_ghost_size=gs, gra_ghost_size=3)
+        with pytest.raises(pg.PgError):  # the gravity stencil needs GRA_GHOST_SIZE <= 2
+            ctx.solve_device(1, 1, 1, 1, 8, pg.solver_args(graacc=True))
