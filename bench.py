@@ -52,7 +52,126 @@ def parse():
     ap.add_argument("--fallback-tol", type=float, default=1e-2,
                     help="guess check: a network guess with relative residual above this restarts from zero")
     ap.add_argument("--mode", choices=["both", "plain", "warm"], default="both")
+    ap.add_argument("--workload", choices=["config2", "config3", "config5"], default="config2",
+                    help="config2 (default): the metric's workload; config3: 32768 mixed-GS lognormal patches, "
+                         "half SOR half MG, one graph-captured persistent launch; config5: 65536 mixed-GS "
+                         "patches (power law / Plummer / lognormal), warm start with guess check, SOR+MG, gravity")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a known-size reference for "
+                         "rocprof FETCH_SIZE / WRITE_SIZE calibration)")
     return ap.parse_args()
+
+
+OMEGA_BY_GS = [1.49, 1.57, 1.62, 1.65, 1.69]
+
+
+def run_mixed(a):
+    """Mixed ghost sizes and schemes in one persistent launch per scheme (configurations 3 and 5)."""
+    import torch
+
+    import gpu_testbed_poisson_gravity_amd as pg
+    from gpu_testbed_poisson_gravity_amd import inputs as I
+
+    c5 = a.workload == "config5"
+    n = a.npatch if a.npatch != 16384 else (65536 if c5 else 32768)
+    kinds = ("powerlaw", "plummer", "lognormal") if c5 else ("lognormal",)
+    t0 = time.time()
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    gs, kind, rho, cpot, flu = I.mixed_batch(n, a.seed, kinds=kinds, workers=workers)
+    sch = np.where(np.random.default_rng(a.seed + 1).permutation(n) % 2 == 0, pg.SOR, pg.MG).astype(np.int32)
+    log(f"[bench] {a.workload}: {n} patches generated in {time.time() - t0:.1f}s")
+    dev = torch.device("cuda", 0)
+    poi = 4.0 * np.pi * 1e-5
+    args = pg.solver_args(dt=1e-3, sor_min_iter=10, sor_max_iter=1000, sor_omega=1.69, mg_max_iter=100, mg_npre=2,
+                          mg_npost=2, mg_tol=1e-5, poi_coeff=poi)
+    omega = np.asarray(OMEGA_BY_GS, np.float32)
+    ctx = pg.Context(0)
+    net = None
+    init = pg.INIT_INTERP
+    if c5 and os.path.exists(a.weights):
+        net = pg.CNN(ctx)
+        net.load_params(a.weights)
+        init = pg.INIT_GUESS
+    ctx.set_options(pot_ghost_size=5, gra_ghost_size=1, init_mode=init, term_mode=pg.TERM_RELRES, rel_tol=1e-5,
+                    check_every=a.check_every, fuse_gravity=1, guess_fallback_tol=a.fallback_tol if net else 0.0)
+    # autotune every ghost size on its own SOR patches (layout table used by the persistent kernel)
+    tuned = {}
+    for g in range(1, 6):
+        idx = np.nonzero((gs == g) & (sch == pg.SOR))[0][:4096]
+        if idx.size == 0:
+            continue
+        d_r = torch.as_tensor(np.stack([rho[i] for i in idx])).to(dev)
+        d_c = torch.as_tensor(np.stack([cpot[i] for i in idx])).to(dev)
+        d_o = torch.empty((idx.size, 10, 10, 10), device=dev)
+        ctx.set_options(pot_ghost_size=g, init_mode=pg.INIT_INTERP)
+        ga = pg.solver_args(dt=1e-3, sor_min_iter=10, sor_max_iter=1000, sor_omega=float(omega[g - 1]), poi_coeff=poi,
+                            graacc=False)
+        tuned[g] = ctx.autotune(g, d_r, d_c, d_o, idx.size, ga, repeats=2)[:2]
+    ctx.set_options(pot_ghost_size=5, init_mode=init)
+    plan = ctx.plan(gs, sch)
+    d_rho = torch.as_tensor(pg.pack(rho, plan.rho_off, plan.rho_floats)).to(dev)
+    d_cpot = torch.as_tensor(pg.pack(cpot, plan.pot_in_off, plan.pot_in_floats)).to(dev)
+    d_flu0 = torch.as_tensor(flu).to(dev)
+    d_flu = d_flu0.clone()
+    d_out = torch.empty((n, 10, 10, 10), device=dev)
+    d_guess = torch.zeros(plan.rho_floats, device=dev) if net is not None else None
+    d_it = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_st = torch.zeros(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    graph = ctx.graph_mixed(plan, d_rho, d_cpot, d_out, d_flu, args, guess=d_guess, omega_by_ghost=omega,
+                            iters=d_it, status=d_st)
+
+    def step(e=None):
+        d_flu.copy_(d_flu0)
+        if e is not None:
+            e[0].record(stream)
+        if net is not None:
+            net.warm_start_mixed(plan, d_rho, d_cpot, args, d_guess, stream=stream.cuda_stream)
+        if e is not None:
+            e[1].record(stream)
+        graph.launch(stream.cuda_stream)
+        if e is not None:
+            e[2].record(stream)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize(dev)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
+    for s_ in range(a.steps):
+        step(evs[s_])
+    torch.cuda.synchronize(dev)
+    pre = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    sol = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    it, st = d_it.cpu().numpy(), d_st.cpu().numpy()
+    R = 6 + 2 * gs.astype(np.float64)
+    sor_lds = float((it * R ** 3 * LDS_BYTES_PER_UPDATE)[sch == pg.SOR].sum())
+    ms = pre + sol
+    out = {
+        "metric": f"patch-solves/s at rel. residual <=1e-5, mixed ghost sizes ({a.workload}), 1 MI355X",
+        "value": n / (ms * 1e-3), "unit": "patch-solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n} patches, GS uniform in 1..5, densities {'/'.join(kinds)}, half SOR (omega by "
+                               f"GS {OMEGA_BY_GS}) half MG (2+2 smoothing), rel. residual 1e-5, fused gravity, one "
+                               f"graph-captured persistent launch per scheme, autotuned layouts {tuned}; start: "
+                               + ("learned warm start (guess check 1e-2)" if net is not None else "interpolation"),
+                   "npatch": n},
+        "network_ms": pre, "solve_ms": sol,
+        "sor": {"mean_iters": float(it[sch == pg.SOR].mean()), "max_iters": int(it[sch == pg.SOR].max())},
+        "mg": {"mean_vcycles": float(it[sch == pg.MG].mean()), "max_vcycles": int(it[sch == pg.MG].max())},
+        "converged": int(((st & pg.STATUS_CONVERGED) != 0).sum()),
+        "fallbacks": int(((st & pg.STATUS_FALLBACK) != 0).sum()),
+        "roofline": {"bound": "lds", "achieved": sor_lds / (sol * 1e-3) / 1e9, "peak": LDS_PEAK / 1e9, "unit": "GB/s",
+                     "frac": sor_lds / (sol * 1e-3) / LDS_PEAK, "traffic": None,
+                     "note": "SOR patches' algorithmic LDS bytes over the whole graph time (MG time included)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    graph.close()
+    plan.close()
+    if net is not None:
+        net.close()
+    ctx.close()
 
 
 def main():
@@ -60,6 +179,9 @@ def main():
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if a.workload != "config2":
+        run_mixed(a)
+        return
     import torch
 
     import gpu_testbed_poisson_gravity_amd as pg
@@ -87,6 +209,12 @@ def main():
         ctx.set_layout(a.gs, a.ppw if a.ppw is not None else ppw0, a.pad if a.pad is not None else pad0)
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
+    if a.copy_mib > 0:
+        src = torch.ones(a.copy_mib * 2 ** 18, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize(dev)
+        del src, dst
 
     def run(init_mode, net):
         """warmup + timed steps; events bracket the network phase and the solve separately."""

@@ -119,6 +119,8 @@ SIGNATURES = {
                                        C.c_void_p]),
     "pgamd_cnn_warm_start": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(SolverArgs),
                                        C.c_void_p, C.c_void_p]),
+    "pgamd_cnn_warm_start_mixed": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.POINTER(SolverArgs), C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -434,3 +436,7 @@ class CNN:
     def warm_start(self, rho, cpot, npatch, args: SolverArgs, guess, stream=None):
         _check(load().pgamd_cnn_warm_start(self.ctx.h, self.h, _vp(rho), _vp(cpot), npatch, C.byref(args),
                                            _vp(guess), stream))
+
+    def warm_start_mixed(self, plan, rho, cpot, args: SolverArgs, guess, stream=None):
+        _check(load().pgamd_cnn_warm_start_mixed(self.ctx.h, self.h, plan.h, _vp(rho), _vp(cpot), C.byref(args),
+                                                 _vp(guess), stream))

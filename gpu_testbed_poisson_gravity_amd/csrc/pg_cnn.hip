@@ -29,6 +29,7 @@
 #include "pg_cnn.h"
 #include "pg_geometry.h"
 #include "pg_interp.h"
+#include "pg_kernels.h"
 
 #pragma clang fp contract(off)
 
@@ -447,6 +448,169 @@ __global__ void __launch_bounds__(kThreads) guess_kernel(const float* __restrict
   }
 }
 
+// ---- fused inference: one workgroup per patch, activations resident in LDS ------------------------------
+// interp + packing -> six convolutions ping-ponging between two padded 10^3 x 16 BF16 buffers -> guess.
+// Each output value goes through exactly the MFMA sequence of conv_fwd_kernel, so the guess is bitwise
+// identical to prep + forward + guess with the layer-by-layer kernels.
+
+// One convolution layer from LDS to LDS (MODE 0/1) or to the FP32 output D in LDS (MODE 2).
+template <int MODE>
+__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const u16* __restrict__ W,
+                                         const float* __restrict__ bias, float* D) {
+  bf16x8 wreg[KC];
+  load_weights(W, wreg);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int co = lane & 15, kq = 8 * (lane >> 4);
+  const float b = bias[co];
+  for (int t = wave; t < kCnnVox / 16; t += kThreads / 64) {
+    const int v = t * 16 + (lane & 15);
+    const int oz = v / (So * So), oy = (v / So) % So, ox = v % So;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < KC; c++) {
+      const int kb = c * 32 + kq;
+      const int tap = kb / CH, ci0 = kb % CH;
+      Pack8 a;
+      if (tap < 27) {
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+        a.u = *reinterpret_cast<const uint4*>(in + (((oz + dz) * S + oy + dy) * S + ox + dx) * CH + ci0);
+      } else {
+        a.u = make_uint4(0, 0, 0, 0);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, wreg[c], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int vo = t * 16 + 4 * (lane >> 4) + r;
+      const int z = vo / (So * So), y = (vo / So) % So, x = vo % So;
+      const float h = acc[r] + b;
+      if (MODE == 2) {
+        if (co < kCnnOut) D[vo * kCnnOut + co] = h;
+      } else {
+        const int c = (((z + 1) * S + y + 1) * S + x + 1) * CH + co;
+        float o = fmaxf(h, 0.0f);
+        if (MODE == 1) o += bf2f(in[c]);
+        out[c] = f2bf(o);
+      }
+    }
+  }
+}
+
+// Zero the border voxels (any coordinate 0 or 9) of a padded activation buffer.
+__device__ __forceinline__ void zero_border(uint4* buf) {
+  for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kThreads) {
+    const int vox = i >> 1;
+    const int x = vox % S, y = (vox / S) % S, z = vox / (S * S);
+    if (x == 0 || x == S - 1 || y == 0 || y == S - 1 || z == 0 || z == S - 1) buf[i] = make_uint4(0, 0, 0, 0);
+  }
+}
+
+template <int GS>
+__device__ void fused_patch(const float* __restrict__ rho, const float* __restrict__ cpot, const u16* __restrict__ Wb,
+                            const float* __restrict__ P, float cst, int int_scheme, float* __restrict__ guess,
+                            uint4* bufA, uint4* bufB) {
+  constexpr Geo g = geo_of(GS);
+  constexpr int N = g.N, R = g.R, OFF = 5 - GS;
+  static_assert(N * N * N * 4 <= kCnnPadVox * CH * 2, "interpolated cube must fit one activation buffer");
+  __shared__ float red[kThreads];
+  float* cube = reinterpret_cast<float*>(bufB);
+  u16* A = reinterpret_cast<u16*>(bufA);
+  u16* B = reinterpret_cast<u16*>(bufB);
+  // input: interpolate into B (as scratch), scale, pack into A
+  interp_cube<GS>(cpot, int_scheme, cube);
+  float mx = 0.0f;
+  for (int t = threadIdx.x; t < N * N * N; t += kThreads) mx = fmaxf(mx, fabsf(cube[t]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const float s = red[0] > 0.0f ? red[0] : 1.0f;
+  const float inv = 1.0f / s;
+  for (int t = threadIdx.x; t < kCnnPadVox * 8; t += kThreads) {
+    const int vox = t >> 3, d = t & 7;
+    const int cx = vox % S, cy = (vox / S) % S, cz = vox / (S * S);
+    const int X = 2 * cx - 1 + (d & 1) - OFF, Y = 2 * cy - 1 + ((d >> 1) & 1) - OFF, Z = 2 * cz - 1 + (d >> 2) - OFF;
+    float c0 = 0.0f, c1 = 0.0f;
+    if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) {
+      c0 = cube[(Z * N + Y) * N + X] * inv;
+      if (X >= 1 && X <= R && Y >= 1 && Y <= R && Z >= 1 && Z <= R)
+        c1 = cst * rho[((Z - 1) * R + (Y - 1)) * R + (X - 1)] * inv;
+    }
+    reinterpret_cast<uint32_t*>(A)[t] = (uint32_t)f2bf(c0) | ((uint32_t)f2bf(c1) << 16);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kThreads) bufB[i] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  const size_t LW = (size_t)16 * K;
+  conv_lds<0>(A, B, Wb, P + 0 * kCnnLayerParams + 16 * K, nullptr);          // layer 1: A -> B
+  __syncthreads();
+  zero_border(bufA);                                                           // A held the input's border
+  conv_lds<1>(B, A, Wb + LW, P + 1 * kCnnLayerParams + 16 * K, nullptr);      // layer 2: B -> A
+  __syncthreads();
+  conv_lds<1>(A, B, Wb + 2 * LW, P + 2 * kCnnLayerParams + 16 * K, nullptr);  // layer 3
+  __syncthreads();
+  conv_lds<1>(B, A, Wb + 3 * LW, P + 3 * kCnnLayerParams + 16 * K, nullptr);  // layer 4
+  __syncthreads();
+  conv_lds<1>(A, B, Wb + 4 * LW, P + 4 * kCnnLayerParams + 16 * K, nullptr);  // layer 5
+  __syncthreads();
+  float* D = reinterpret_cast<float*>(bufA);
+  conv_lds<2>(B, nullptr, Wb + 5 * LW, P + 5 * kCnnLayerParams + 16 * K, D);  // layer 6: B -> D (in A)
+  __syncthreads();
+  interp_cube<GS>(cpot, int_scheme, cube);                                     // B is free again
+  for (int t = threadIdx.x; t < R * R * R; t += kThreads) {
+    const int x = t % R, y = (t / R) % R, z = t / (R * R);
+    const int ix = x + OFF, iy = y + OFF, iz = z + OFF;
+    const int o = ((iz >> 1) * So + (iy >> 1)) * So + (ix >> 1);
+    const int p = ((iz & 1) * 2 + (iy & 1)) * 2 + (ix & 1);
+    guess[t] = cube[((z + 1) * N + y + 1) * N + x + 1] + s * D[o * kCnnOut + p];
+  }
+}
+
+__device__ __forceinline__ void fused_dispatch(int gs, const float* r, const float* c, const u16* Wb, const float* P,
+                                               float cst, int int_scheme, float* out, uint4* bufA, uint4* bufB) {
+  switch (gs) {
+    case 1: fused_patch<1>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
+    case 2: fused_patch<2>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
+    case 3: fused_patch<3>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
+    case 4: fused_patch<4>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
+    default: fused_patch<5>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) fused_guess_kernel(const float* __restrict__ rho,
+                                                               const float* __restrict__ cpot,
+                                                               const u16* __restrict__ Wb, const float* __restrict__ P,
+                                                               int npatch, int gs, float cst, int int_scheme,
+                                                               float* __restrict__ guess) {
+  __shared__ uint4 bufA[kCnnPadVox * 2];
+  __shared__ uint4 bufB[kCnnPadVox * 2];
+  const int p = blockIdx.x;
+  if (p >= npatch) return;
+  const Geo g = geo_of(gs);
+  const float* r = rho + (size_t)p * g.R * g.R * g.R;
+  const float* c = cpot + (size_t)p * g.C * g.C * g.C;
+  float* out = guess + (size_t)p * g.R * g.R * g.R;
+  fused_dispatch(gs, r, c, Wb, P, cst, int_scheme, out, bufA, bufB);
+}
+
+// Mixed ghost sizes: patch p reads its ghost size and packed offsets from the plan's descriptors; the
+// guess is packed like the density (offset rho_off).
+__global__ void __launch_bounds__(kThreads) fused_guess_mixed_kernel(const float* __restrict__ rho,
+                                                                     const float* __restrict__ cpot,
+                                                                     const PatchDesc* __restrict__ desc,
+                                                                     const u16* __restrict__ Wb,
+                                                                     const float* __restrict__ P, int npatch, float cst,
+                                                                     int int_scheme, float* __restrict__ guess) {
+  __shared__ uint4 bufA[kCnnPadVox * 2];
+  __shared__ uint4 bufB[kCnnPadVox * 2];
+  const int p = blockIdx.x;
+  if (p >= npatch) return;
+  const PatchDesc d = desc[p];
+  fused_dispatch(d.gs, rho + d.rho_off, cpot + d.cpot_off, Wb, P, cst, int_scheme, guess + d.rho_off, bufA, bufB);
+}
+
 }  // namespace cnn
 
 // ---- host launchers ----------------------------------------------------------------------------------------
@@ -513,6 +677,20 @@ hipError_t cnn_prep(const float* rho, const float* cpot, int n, int gs, float cs
   const Geo g = geo_of(gs);
   hipLaunchKernelGGL(prep_kernel, dim3(n), dim3(kThreads), sizeof(float) * g.N * g.N * g.N, s, rho, cpot, gs, cst,
                      int_scheme, x, scale);
+  return hipGetLastError();
+}
+
+hipError_t cnn_fused_guess(const float* rho, const float* cpot, const uint16_t* Wb, const float* P, int n, int gs,
+                           float cst, int int_scheme, float* guess, hipStream_t s) {
+  hipLaunchKernelGGL(fused_guess_kernel, dim3(n), dim3(kThreads), 0, s, rho, cpot, Wb, P, n, gs, cst, int_scheme,
+                     guess);
+  return hipGetLastError();
+}
+
+hipError_t cnn_fused_guess_mixed(const float* rho, const float* cpot, const PatchDesc* desc, const uint16_t* Wb,
+                                 const float* P, int n, float cst, int int_scheme, float* guess, hipStream_t s) {
+  hipLaunchKernelGGL(fused_guess_mixed_kernel, dim3(n), dim3(kThreads), 0, s, rho, cpot, desc, Wb, P, n, cst,
+                     int_scheme, guess);
   return hipGetLastError();
 }
 

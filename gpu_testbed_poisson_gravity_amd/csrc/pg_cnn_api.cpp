@@ -47,10 +47,6 @@ struct pgamd_cnn {
   uint16_t* dh = nullptr;   // [n][8^3][16]
   float *d = nullptr, *dy6 = nullptr, *dyA = nullptr, *dyB = nullptr, *part = nullptr, *lossp = nullptr;
   size_t part_floats = 0;
-  // inference
-  int icap = 0;
-  uint16_t* x0 = nullptr;
-  float* scale = nullptr;
 };
 
 namespace {
@@ -218,8 +214,6 @@ int pgamd_cnn_destroy(pgamd_cnn* n) {
   hipFree(n->v);
   hipFree(n->wb);
   hipFree(n->wt);
-  hipFree(n->x0);
-  hipFree(n->scale);
   delete n;
   return PGAMD_OK;
 }
@@ -305,27 +299,29 @@ int pgamd_cnn_warm_start(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho, c
                          const pgamd_solver_args* a, float* d_guess, void* stream) {
   if (!ctx || !n || !a || !d_rho || !d_pot_in || !d_guess || npatch <= 0)
     return set_error(PGAMD_ERR_ARG, "bad warm-start arguments");
+  if (a->int_scheme != PGAMD_INT_CQUAD && a->int_scheme != PGAMD_INT_QUAD)
+    return set_error(PGAMD_ERR_ARG, "incorrect parameter IntScheme");
   const int gs = ctx_options(ctx).pot_ghost_size;
   hipSetDevice(n->device);
-  hipStream_t s = (hipStream_t)stream;
-  if (npatch > n->icap) {
-    hipFree(n->x0);
-    hipFree(n->scale);
-    n->x0 = nullptr;
-    n->scale = nullptr;
-    n->icap = 0;
-    if (hipMalloc(&n->x0, (size_t)npatch * kCnnInput * 2) != hipSuccess ||
-        hipMalloc(&n->scale, (size_t)npatch * 4) != hipSuccess)
-      return set_error(PGAMD_ERR_NOMEM, "warm-start workspace allocation failed");
-    n->icap = npatch;
-  }
-  int rc = pgamd_cnn_prepare(ctx, d_rho, d_pot_in, npatch, gs, a, n->x0, n->scale, stream);
-  if (rc != PGAMD_OK) return rc;
-  rc = ensure_ws(n, npatch, n->train_ws && npatch <= n->cap);
-  if (rc != PGAMD_OK) return rc;
-  rc = forward(n, n->x0, npatch, false, s);
-  if (rc != PGAMD_OK) return rc;
-  CK(cnn_guess(d_pot_in, n->d, n->scale, npatch, gs, a->int_scheme, d_guess, s));
+  const float cst = a->poi_coeff * a->dh * a->dh;
+  // one launch: interpolation, packing, the six layers in LDS and the guess
+  CK(cnn_fused_guess(d_rho, d_pot_in, n->wb, n->p, npatch, gs, cst, a->int_scheme, d_guess, (hipStream_t)stream));
+  return PGAMD_OK;
+}
+
+int pgamd_cnn_warm_start_mixed(pgamd_context* ctx, pgamd_cnn* n, pgamd_plan* plan, const float* d_rho,
+                               const float* d_pot_in, const pgamd_solver_args* a, float* d_guess, void* stream) {
+  if (!ctx || !n || !plan || !a || !d_rho || !d_pot_in || !d_guess)
+    return set_error(PGAMD_ERR_ARG, "bad warm-start arguments");
+  if (plan_device(plan) != n->device) return set_error(PGAMD_ERR_ARG, "plan and network belong to different devices");
+  if (a->int_scheme != PGAMD_INT_CQUAD && a->int_scheme != PGAMD_INT_QUAD)
+    return set_error(PGAMD_ERR_ARG, "incorrect parameter IntScheme");
+  const int np = plan_npatch(plan);
+  if (np == 0) return PGAMD_OK;
+  hipSetDevice(n->device);
+  const float cst = a->poi_coeff * a->dh * a->dh;
+  CK(cnn_fused_guess_mixed(d_rho, d_pot_in, plan_device_desc(plan), n->wb, n->p, np, cst, a->int_scheme, d_guess,
+                           (hipStream_t)stream));
   return PGAMD_OK;
 }
 

@@ -25,6 +25,11 @@ hipStream_t ctx_stream(const pgamd_context* c);
 hipStream_t ctx_aux_stream(pgamd_context* c);
 hipStream_t ctx_capture_stream(pgamd_context* c);
 hipEvent_t ctx_event(pgamd_context* c, int i);
+// Plan internals for the other modules.
+const PatchDesc* plan_device_desc(const pgamd_plan* pl);
+int plan_npatch(const pgamd_plan* pl);
+int plan_device(const pgamd_plan* pl);
+bool plan_has_ghost(const pgamd_plan* pl, int gs);
 // Frees whatever the optional modules (graphs, mixed batches, CNN) attached to a context.
 void release_extras(pgamd_context* c);
 

@@ -41,6 +41,14 @@ struct pgamd_graph {
   hipGraphExec_t exec = nullptr;
 };
 
+namespace pgamd {
+const PatchDesc* plan_device_desc(const pgamd_plan* pl) { return pl->d_desc; }
+int plan_npatch(const pgamd_plan* pl) { return pl->npatch; }
+int plan_device(const pgamd_plan* pl) { return pl->device; }
+bool plan_has_ghost(const pgamd_plan* pl, int gs) { return gs >= 1 && gs <= kMaxGhost && pl->has_gs[gs]; }
+}  // namespace pgamd
+
+
 namespace {
 
 void free_plan(pgamd_plan* p) {

@@ -245,3 +245,46 @@ def uniform_quadratic(npatch: int, gs: int, rho_c: float = 1.0, dh: float = 1.0,
     cpot = (2.0 / 3.0) * np.pi * newton_g * rho_c * _r2(coarse_centres(gs, dh), centre)
     flu = _fluid_from(_patch_proper(rho, gs), 1.0)
     return rho.astype(dtype), cpot.astype(dtype), flu.astype(dtype)
+
+
+_KINDS = {"powerlaw": lambda n, gs, seed: softened_powerlaw(n, gs, seed),
+          "plummer": lambda n, gs, seed: plummer(n, gs, seed),
+          "lognormal": lambda n, gs, seed: lognormal(n, gs, seed, chunk=128)}
+
+
+def _mixed_job(job):
+    kind, gs, n, seed = job
+    r, c, f, _ = _KINDS[kind](n, gs, seed)
+    return r, c, f
+
+
+def mixed_batch(npatch: int, seed: int, kinds=("lognormal",), gs_choices=(1, 2, 3, 4, 5), workers: int = 1,
+                chunk: int = 256):
+    """Seeded batch of npatch patches with ghost sizes drawn uniformly from gs_choices and density types
+    assigned in equal shares (kinds cycled over a seeded permutation).  Returns (gs [n], kind [n], rho list,
+    cpot list, flu [n,5,8,8,8]); generation of the (kind, gs) groups runs on `workers` host processes."""
+    rng = np.random.default_rng(seed)
+    gs = rng.choice(np.asarray(gs_choices, np.int32), npatch)
+    kind = np.asarray(kinds)[rng.permutation(np.arange(npatch) % len(kinds))]
+    jobs, where = [], []
+    for ki, k in enumerate(kinds):
+        for g in gs_choices:
+            idx = np.nonzero((gs == g) & (kind == k))[0]
+            for j, b in enumerate(range(0, idx.size, chunk)):
+                sel = idx[b:b + chunk]
+                jobs.append((k, int(g), int(sel.size), seed * 7919 + ki * 1000 + g * 100 + j))
+                where.append(sel)
+    if workers > 1 and len(jobs) > 1:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(min(workers, len(jobs))) as pool:
+            parts = pool.map(_mixed_job, jobs, chunksize=1)
+    else:
+        parts = [_mixed_job(j) for j in jobs]
+    rho, cpot = [None] * npatch, [None] * npatch
+    flu = np.zeros((npatch, 5, PATCH, PATCH, PATCH), np.float32)
+    for sel, (r, c, f) in zip(where, parts):
+        for k, i in enumerate(sel):
+            rho[i], cpot[i] = r[k], c[k]
+        flu[sel] = f
+    return gs, kind, rho, cpot, flu

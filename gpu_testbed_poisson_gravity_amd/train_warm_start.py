@@ -175,8 +175,14 @@ def main(argv=None):
     print(f"[train] targets: solver iterations mean {it_t.mean():.1f} max {it_t.max()}, converged "
           f"{int((st_t & 1).sum())}/{it_t.size}, target rms {float(target.square().mean().sqrt()):.3e}",
           flush=True)
+    # train on targets normalised by their RMS; the output layer is linear, so the scale folds back into it
+    tscale = float(target.square().mean().sqrt())
+    target /= tscale
+    o6 = 5 * (16 * 432 + 16)              # output layer W6, b6
     if a.init:
-        net.load_params(a.init)
+        p = np.load(a.init, allow_pickle=False).astype(np.float32)
+        p[o6:] /= tscale
+        net.set_params(p)
 
     n = x0.shape[0]
     gen = torch.Generator(device="cuda")
@@ -187,18 +193,25 @@ def main(argv=None):
         idx = torch.randint(0, n, (a.batch,), device="cuda", generator=gen)
         xb, tb = x0.index_select(0, idx), target.index_select(0, idx)
         want = step % 100 == 0 or step == a.steps - 1
-        loss = net.train_step(xb, tb, a.batch, a.lr, want_loss=want)
+        # constant rate, then a linear decay to a tenth over the last quarter of the steps
+        frac = step / max(a.steps - 1, 1)
+        lr = a.lr if frac < 0.75 else a.lr * (1.0 - 0.9 * (frac - 0.75) / 0.25)
+        loss = net.train_step(xb, tb, a.batch, lr, want_loss=want)
         done = step + 1
         if want:
             ema = loss if ema is None else 0.9 * ema + 0.1 * loss
             if step % 500 == 0 or step == a.steps - 1:
                 curve.append([step, loss])
-                print(f"[train] step {step} loss {loss:.4e} (ema {ema:.4e}) {time.time() - t0:.0f} s", flush=True)
+                print(f"[train] step {step} loss {loss:.4e} (normalised; ema {ema:.4e}) lr {lr:.1e} "
+                      f"{time.time() - t0:.0f} s", flush=True)
         if time.time() - t0 > a.time_limit:
             print(f"[train] time limit reached after {done} steps", flush=True)
             break
     torch.cuda.synchronize()
     train_s = time.time() - t0
+    p = net.get_params()
+    p[o6:] *= tscale                      # undo the target normalisation
+    net.set_params(p)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     net.save(a.out)
     del x0, target
@@ -211,6 +224,7 @@ def main(argv=None):
               f"cnn {k['mean_iters']:.1f} (fallbacks {k['fallbacks']}); time zero {z['ms']:.2f} ms cnn "
               f"{k['ms']:.2f} ms", flush=True)
     record = {"patches": n, "lognormal_frac": a.lognormal_frac, "steps": done, "batch": a.batch, "lr": a.lr,
+              "lr_schedule": "constant, then linear decay to lr/10 over the last quarter", "target_rms": tscale,
               "seed": a.seed, "train_seconds": train_s, "ms_per_step": 1e3 * train_s / max(done, 1),
               "loss_curve": curve, "target_solver": {"rel_tol": 1e-6, "mean_iters": float(it_t.mean())},
               "eval_patches": a.eval_patches, "eval": ev, "device": ctx.device_info()}

@@ -335,3 +335,48 @@ def test_autotune_keeps_results_and_roundtrips(tmp_path):
         c2.autotune_load(path)
         assert c2.get_layout(gs) == (ppw, pad)
     assert os.path.exists(path)
+
+
+def test_mixed_warm_start_matches_uniform():
+    """pgamd_cnn_warm_start_mixed on a plan equals pgamd_cnn_warm_start per ghost size, bit for bit."""
+    torch = _torch()
+    rng = np.random.default_rng(12)
+    gsz = rng.integers(1, 6, 40).astype(np.int32)
+    rho, cpot = [None] * 40, [None] * 40
+    for g in range(1, 6):
+        idx = np.nonzero(gsz == g)[0]
+        if idx.size:
+            r, c, _, _ = I.plummer(idx.size, g, seed=60 + g)
+            for k, i in enumerate(idx):
+                rho[i], cpot[i] = r[k], c[k]
+    args = pg.solver_args(dh=1.0, poi_coeff=POI)
+    p = _random_params(21)
+    with pg.Context(0) as c:
+        net = pg.CNN(c)
+        plan = c.plan(gsz)
+        try:
+            net.set_params(p)
+            d_rho = _dev(pg.pack(rho, plan.rho_off, plan.rho_floats))
+            d_cpot = _dev(pg.pack(cpot, plan.pot_in_off, plan.pot_in_floats))
+            d_guess = torch.zeros(plan.rho_floats, device="cuda")
+            print("[case] mixed warm start", flush=True)
+            net.warm_start_mixed(plan, d_rho, d_cpot, args, d_guess)
+            torch.cuda.synchronize()
+            got = d_guess.cpu().numpy()
+            for g in range(1, 6):
+                idx = np.nonzero(gsz == g)[0]
+                if idx.size == 0:
+                    continue
+                R = pg.geometry(g)["R"]
+                c.set_options(pot_ghost_size=g)
+                u = torch.empty((idx.size, R, R, R), device="cuda")
+                net.warm_start(_dev(np.stack([rho[i] for i in idx])), _dev(np.stack([cpot[i] for i in idx])),
+                               idx.size, args, u)
+                torch.cuda.synchronize()
+                u = u.cpu().numpy()
+                for k, i in enumerate(idx):
+                    o = plan.rho_off[i]
+                    assert np.array_equal(got[o:o + R ** 3].reshape(R, R, R), u[k]), (g, i)
+        finally:
+            plan.close()
+            net.close()
