@@ -452,28 +452,47 @@ __global__ void __launch_bounds__(kThreads) guess_kernel(const float* __restrict
 // interp + packing -> six convolutions ping-ponging between two padded 10^3 x 16 BF16 buffers -> guess.
 // Each output value goes through exactly the MFMA sequence of conv_fwd_kernel, so the guess is bitwise
 // identical to prep + forward + guess with the layer-by-layer kernels.
+//
+// * An activation buffer is split into two half-planes of 8 channels, [half][voxel][8] (16 KB each), so
+//   that the 16-byte A-operand reads of 8 consecutive lanes (8 consecutive voxels along x) are one
+//   contiguous 128-byte span: no bank conflicts.
+// * Global traffic is issued early and coalesced: the coarse cube is staged once (float4) and kept in LDS
+//   for the final interpolation, the density is read with float4 loads and scattered into the packed
+//   input, and each layer's weights are loaded into registers while the previous layer computes.
+
+constexpr int kHalf = kCnnPadVox * 8;        // bf16 values per half-plane
+constexpr int kCoarseMax = 12 * 12 * 12;     // C^3 at ghost size 5
+
+struct FusedLds {
+  uint4 A[kCnnPadVox * 2];
+  uint4 B[kCnnPadVox * 2];
+  float cs[kCoarseMax];
+  float red[kThreads / 64];
+};
+
+__device__ __forceinline__ void load_layer(const u16* __restrict__ Wb, const float* __restrict__ P, int l,
+                                           bf16x8 (&w)[KC], float& b) {
+  load_weights(Wb + (size_t)l * 16 * K, w);
+  b = P[(size_t)l * kCnnLayerParams + 16 * K + (threadIdx.x & 15)];
+}
 
 // One convolution layer from LDS to LDS (MODE 0/1) or to the FP32 output D in LDS (MODE 2).
 template <int MODE>
-__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const u16* __restrict__ W,
-                                         const float* __restrict__ bias, float* D) {
-  bf16x8 wreg[KC];
-  load_weights(W, wreg);
+__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KC], float b, float* D) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int co = lane & 15, kq = 8 * (lane >> 4);
-  const float b = bias[co];
   for (int t = wave; t < kCnnVox / 16; t += kThreads / 64) {
     const int v = t * 16 + (lane & 15);
-    const int oz = v / (So * So), oy = (v / So) % So, ox = v % So;
+    const int base = ((v >> 6) * S + ((v >> 3) & 7)) * S + (v & 7);   // padded index of tap (0,0,0)
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < KC; c++) {
       const int kb = c * 32 + kq;
-      const int tap = kb / CH, ci0 = kb % CH;
+      const int tap = kb / CH, half = (kb % CH) >> 3;
       Pack8 a;
       if (tap < 27) {
-        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-        a.u = *reinterpret_cast<const uint4*>(in + (((oz + dz) * S + oy + dy) * S + ox + dx) * CH + ci0);
+        const int off = ((tap / 9) * S + (tap / 3) % 3) * S + tap % 3;
+        a.u = *reinterpret_cast<const uint4*>(in + half * kHalf + (base + off) * 8);
       } else {
         a.u = make_uint4(0, 0, 0, 0);
       }
@@ -482,12 +501,12 @@ __device__ __forceinline__ void conv_lds(const u16* in, u16* out, const u16* __r
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int vo = t * 16 + 4 * (lane >> 4) + r;
-      const int z = vo / (So * So), y = (vo / So) % So, x = vo % So;
       const float h = acc[r] + b;
       if (MODE == 2) {
         if (co < kCnnOut) D[vo * kCnnOut + co] = h;
       } else {
-        const int c = (((z + 1) * S + y + 1) * S + x + 1) * CH + co;
+        const int q = (((vo >> 6) + 1) * S + ((vo >> 3) & 7) + 1) * S + (vo & 7) + 1;
+        const int c = (co >> 3) * kHalf + q * 8 + (co & 7);
         float o = fmaxf(h, 0.0f);
         if (MODE == 1) o += bf2f(in[c]);
         out[c] = f2bf(o);
@@ -496,70 +515,126 @@ __device__ __forceinline__ void conv_lds(const u16* in, u16* out, const u16* __r
   }
 }
 
-// Zero the border voxels (any coordinate 0 or 9) of a padded activation buffer.
+// Zero the border voxels (any coordinate 0 or 9) of both half-planes of a padded activation buffer.
 __device__ __forceinline__ void zero_border(uint4* buf) {
   for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kThreads) {
-    const int vox = i >> 1;
+    const int vox = i % kCnnPadVox;
     const int x = vox % S, y = (vox / S) % S, z = vox / (S * S);
     if (x == 0 || x == S - 1 || y == 0 || y == S - 1 || z == 0 || z == S - 1) buf[i] = make_uint4(0, 0, 0, 0);
   }
 }
 
+// Interpolated solve cube from a coarse cube already staged in LDS (same arithmetic as interp_cube).
+template <int GS>
+__device__ void interp_cube_lds(const float* cs, int int_scheme, float* cube) {
+  constexpr Geo g = geo_of(GS);
+  constexpr int N = g.N, C = g.C, U = g.U, NC = C - 2;
+  for (int q = threadIdx.x; q < NC * NC * NC; q += kThreads) {
+    const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
+    float v[3][3][3];
+    for (int dk = 0; dk < 3; dk++)
+      for (int dj = 0; dj < 3; dj++)
+        for (int di = 0; di < 3; di++) v[dk][dj][di] = cs[((k + dk - 1) * C + (j + dj - 1)) * C + (i + di - 1)];
+    float ch[2][2][2];
+    interp_children(v, int_scheme, ch);
+    for (int cz = 0; cz < 2; cz++)
+      for (int cy = 0; cy < 2; cy++)
+        for (int cx = 0; cx < 2; cx++) {
+          const int X = 2 * (i - 1) + cx - U, Y = 2 * (j - 1) + cy - U, Z = 2 * (k - 1) + cz - U;
+          if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) cube[(Z * N + Y) * N + X] = ch[cz][cy][cx];
+        }
+  }
+  __syncthreads();
+}
+
 template <int GS>
 __device__ void fused_patch(const float* __restrict__ rho, const float* __restrict__ cpot, const u16* __restrict__ Wb,
                             const float* __restrict__ P, float cst, int int_scheme, float* __restrict__ guess,
-                            uint4* bufA, uint4* bufB) {
+                            FusedLds& L) {
   constexpr Geo g = geo_of(GS);
-  constexpr int N = g.N, R = g.R, OFF = 5 - GS;
+  constexpr int N = g.N, R = g.R, C = g.C, OFF = 5 - GS;
+  constexpr int R3 = R * R * R, C3 = C * C * C;
   static_assert(N * N * N * 4 <= kCnnPadVox * CH * 2, "interpolated cube must fit one activation buffer");
-  __shared__ float red[kThreads];
-  float* cube = reinterpret_cast<float*>(bufB);
-  u16* A = reinterpret_cast<u16*>(bufA);
-  u16* B = reinterpret_cast<u16*>(bufB);
-  // input: interpolate into B (as scratch), scale, pack into A
-  interp_cube<GS>(cpot, int_scheme, cube);
+  static_assert(C3 <= kCoarseMax && R3 % 4 == 0 && C3 % 4 == 0, "float4 staging");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float* cube = reinterpret_cast<float*>(L.B);
+  u16* A = reinterpret_cast<u16*>(L.A);
+  u16* B = reinterpret_cast<u16*>(L.B);
+  // first layer's weights, the coarse cube and this lane's share of the density are requested together
+  bf16x8 wa[KC], wb[KC];
+  float ba, bb;
+  load_layer(Wb, P, 0, wa, ba);
+  for (int t = threadIdx.x; t < C3 / 4; t += kThreads)
+    reinterpret_cast<float4*>(L.cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
+  constexpr int RV = (R3 / 4 + kThreads - 1) / kThreads;   // float4 of density per thread
+  float4 rv[RV];
+#pragma unroll
+  for (int k = 0; k < RV; k++) {
+    const int i4 = threadIdx.x + k * kThreads;
+    rv[k] = i4 < R3 / 4 ? reinterpret_cast<const float4*>(rho)[i4] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  __syncthreads();
+  interp_cube_lds<GS>(L.cs, int_scheme, cube);
   float mx = 0.0f;
   for (int t = threadIdx.x; t < N * N * N; t += kThreads) mx = fmaxf(mx, fabsf(cube[t]));
-  red[threadIdx.x] = mx;
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  if (lane == 0) L.red[wave] = mx;
   __syncthreads();
-  for (int o = kThreads / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
-    __syncthreads();
-  }
-  const float s = red[0] > 0.0f ? red[0] : 1.0f;
+  mx = fmaxf(fmaxf(L.red[0], L.red[1]), fmaxf(L.red[2], L.red[3]));
+  const float s = mx > 0.0f ? mx : 1.0f;
   const float inv = 1.0f / s;
+  // packed input, pass 1: interpolated potential (low halves), zero density (high halves)
+  uint32_t* A32 = reinterpret_cast<uint32_t*>(A);
   for (int t = threadIdx.x; t < kCnnPadVox * 8; t += kThreads) {
     const int vox = t >> 3, d = t & 7;
     const int cx = vox % S, cy = (vox / S) % S, cz = vox / (S * S);
     const int X = 2 * cx - 1 + (d & 1) - OFF, Y = 2 * cy - 1 + ((d >> 1) & 1) - OFF, Z = 2 * cz - 1 + (d >> 2) - OFF;
-    float c0 = 0.0f, c1 = 0.0f;
-    if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) {
-      c0 = cube[(Z * N + Y) * N + X] * inv;
-      if (X >= 1 && X <= R && Y >= 1 && Y <= R && Z >= 1 && Z <= R)
-        c1 = cst * rho[((Z - 1) * R + (Y - 1)) * R + (X - 1)] * inv;
-    }
-    reinterpret_cast<uint32_t*>(A)[t] = (uint32_t)f2bf(c0) | ((uint32_t)f2bf(c1) << 16);
+    float c0 = 0.0f;
+    if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) c0 = cube[(Z * N + Y) * N + X] * inv;
+    // channels 2d, 2d+1 live in half-plane d/4 at in-half channels 2(d%4), 2(d%4)+1
+    A32[((d >> 2) * kCnnPadVox + vox) * 4 + (d & 3)] = (uint32_t)f2bf(c0);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kThreads) bufB[i] = make_uint4(0, 0, 0, 0);
+  // pass 2: Const*rho/s into the high halves (interior cells only), from the density read above
+#pragma unroll
+  for (int k = 0; k < RV; k++) {
+    const int i4 = threadIdx.x + k * kThreads;
+    if (i4 < R3 / 4) {
+      const float e4[4] = {rv[k].x, rv[k].y, rv[k].z, rv[k].w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int idx = 4 * i4 + e;
+        // interior cell -> extended frame index 2 + local + OFF -> coarse voxel (>>1), sub-cell (&1)
+        const int ex = idx % R + 2 + OFF, ey = (idx / R) % R + 2 + OFF, ez = idx / (R * R) + 2 + OFF;
+        const int vox = ((ez >> 1) * S + (ey >> 1)) * S + (ex >> 1);
+        const int d = ((ez & 1) * 2 + (ey & 1)) * 2 + (ex & 1);
+        A[(((d >> 2) * kCnnPadVox + vox) * 4 + (d & 3)) * 2 + 1] = f2bf(cst * e4[e] * inv);
+      }
+    }
+  }
+  for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kThreads) L.B[i] = make_uint4(0, 0, 0, 0);
   __syncthreads();
-  const size_t LW = (size_t)16 * K;
-  conv_lds<0>(A, B, Wb, P + 0 * kCnnLayerParams + 16 * K, nullptr);          // layer 1: A -> B
+  load_layer(Wb, P, 1, wb, bb);
+  conv_lds<0>(A, B, wa, ba, nullptr);          // layer 1: A -> B
   __syncthreads();
-  zero_border(bufA);                                                           // A held the input's border
-  conv_lds<1>(B, A, Wb + LW, P + 1 * kCnnLayerParams + 16 * K, nullptr);      // layer 2: B -> A
+  zero_border(L.A);                            // A held the input's border
+  load_layer(Wb, P, 2, wa, ba);
+  conv_lds<1>(B, A, wb, bb, nullptr);          // layer 2: B -> A
   __syncthreads();
-  conv_lds<1>(A, B, Wb + 2 * LW, P + 2 * kCnnLayerParams + 16 * K, nullptr);  // layer 3
+  load_layer(Wb, P, 3, wb, bb);
+  conv_lds<1>(A, B, wa, ba, nullptr);          // layer 3
   __syncthreads();
-  conv_lds<1>(B, A, Wb + 3 * LW, P + 3 * kCnnLayerParams + 16 * K, nullptr);  // layer 4
+  load_layer(Wb, P, 4, wa, ba);
+  conv_lds<1>(B, A, wb, bb, nullptr);          // layer 4
   __syncthreads();
-  conv_lds<1>(A, B, Wb + 4 * LW, P + 4 * kCnnLayerParams + 16 * K, nullptr);  // layer 5
+  load_layer(Wb, P, 5, wb, bb);
+  conv_lds<1>(A, B, wa, ba, nullptr);          // layer 5
   __syncthreads();
-  float* D = reinterpret_cast<float*>(bufA);
-  conv_lds<2>(B, nullptr, Wb + 5 * LW, P + 5 * kCnnLayerParams + 16 * K, D);  // layer 6: B -> D (in A)
+  float* D = reinterpret_cast<float*>(L.A);
+  conv_lds<2>(B, nullptr, wb, bb, D);          // layer 6: B -> D (in A)
   __syncthreads();
-  interp_cube<GS>(cpot, int_scheme, cube);                                     // B is free again
-  for (int t = threadIdx.x; t < R * R * R; t += kThreads) {
+  interp_cube_lds<GS>(L.cs, int_scheme, cube);  // B is free again
+  for (int t = threadIdx.x; t < R3; t += kThreads) {
     const int x = t % R, y = (t / R) % R, z = t / (R * R);
     const int ix = x + OFF, iy = y + OFF, iz = z + OFF;
     const int o = ((iz >> 1) * So + (iy >> 1)) * So + (ix >> 1);
@@ -569,13 +644,13 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
 }
 
 __device__ __forceinline__ void fused_dispatch(int gs, const float* r, const float* c, const u16* Wb, const float* P,
-                                               float cst, int int_scheme, float* out, uint4* bufA, uint4* bufB) {
+                                               float cst, int int_scheme, float* out, FusedLds& L) {
   switch (gs) {
-    case 1: fused_patch<1>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
-    case 2: fused_patch<2>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
-    case 3: fused_patch<3>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
-    case 4: fused_patch<4>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
-    default: fused_patch<5>(r, c, Wb, P, cst, int_scheme, out, bufA, bufB); break;
+    case 1: fused_patch<1>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 2: fused_patch<2>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 3: fused_patch<3>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 4: fused_patch<4>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    default: fused_patch<5>(r, c, Wb, P, cst, int_scheme, out, L); break;
   }
 }
 
@@ -584,15 +659,14 @@ __global__ void __launch_bounds__(kThreads) fused_guess_kernel(const float* __re
                                                                const u16* __restrict__ Wb, const float* __restrict__ P,
                                                                int npatch, int gs, float cst, int int_scheme,
                                                                float* __restrict__ guess) {
-  __shared__ uint4 bufA[kCnnPadVox * 2];
-  __shared__ uint4 bufB[kCnnPadVox * 2];
+  __shared__ FusedLds L;
   const int p = blockIdx.x;
   if (p >= npatch) return;
   const Geo g = geo_of(gs);
   const float* r = rho + (size_t)p * g.R * g.R * g.R;
   const float* c = cpot + (size_t)p * g.C * g.C * g.C;
   float* out = guess + (size_t)p * g.R * g.R * g.R;
-  fused_dispatch(gs, r, c, Wb, P, cst, int_scheme, out, bufA, bufB);
+  fused_dispatch(gs, r, c, Wb, P, cst, int_scheme, out, L);
 }
 
 // Mixed ghost sizes: patch p reads its ghost size and packed offsets from the plan's descriptors; the
@@ -603,12 +677,11 @@ __global__ void __launch_bounds__(kThreads) fused_guess_mixed_kernel(const float
                                                                      const u16* __restrict__ Wb,
                                                                      const float* __restrict__ P, int npatch, float cst,
                                                                      int int_scheme, float* __restrict__ guess) {
-  __shared__ uint4 bufA[kCnnPadVox * 2];
-  __shared__ uint4 bufB[kCnnPadVox * 2];
+  __shared__ FusedLds L;
   const int p = blockIdx.x;
   if (p >= npatch) return;
   const PatchDesc d = desc[p];
-  fused_dispatch(d.gs, rho + d.rho_off, cpot + d.cpot_off, Wb, P, cst, int_scheme, guess + d.rho_off, bufA, bufB);
+  fused_dispatch(d.gs, rho + d.rho_off, cpot + d.cpot_off, Wb, P, cst, int_scheme, guess + d.rho_off, L);
 }
 
 }  // namespace cnn
