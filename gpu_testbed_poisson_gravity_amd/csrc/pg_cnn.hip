@@ -41,13 +41,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16;
 
 __device__ __forceinline__ float bf2f(u16 v) { return __uint_as_float(((uint32_t)v) << 16); }
-// round to nearest even
-__device__ __forceinline__ u16 f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (u16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (u16)(u >> 16);
-}
+// round to nearest even (one v_cvt_pk_bf16_f32 on gfx950); every kernel here uses this one conversion
+__device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
 
 union Pack8 {
   bf16x8 v;
@@ -460,12 +455,15 @@ __global__ void __launch_bounds__(kThreads) guess_kernel(const float* __restrict
 //   for the final interpolation, the density is read with float4 loads and scattered into the packed
 //   input, and each layer's weights are loaded into registers while the previous layer computes.
 
-constexpr int kHalf = kCnnPadVox * 8;        // bf16 values per half-plane
+// Half-plane stride: 1000 voxels x 8 channels plus 64 bytes, so that channels 0-7 and 8-15 of a voxel
+// fall in different banks (the 2-byte output writes of the two channel halves do not collide).
+constexpr int kHalfP = kCnnPadVox * 8 + 32;  // bf16 values per half-plane (with padding)
+constexpr int kBufVec = kHalfP * 2 / 8;      // uint4 per activation buffer
 constexpr int kCoarseMax = 12 * 12 * 12;     // C^3 at ghost size 5
 
 struct FusedLds {
-  uint4 A[kCnnPadVox * 2];
-  uint4 B[kCnnPadVox * 2];
+  uint4 A[kBufVec];
+  uint4 B[kBufVec];
   float cs[kCoarseMax];
   float red[kThreads / 64];
 };
@@ -476,40 +474,56 @@ __device__ __forceinline__ void load_layer(const u16* __restrict__ Wb, const flo
   b = P[(size_t)l * kCnnLayerParams + 16 * K + (threadIdx.x & 15)];
 }
 
+// This lane's A-operand offset (bf16 elements, relative to the tile voxel's padded index) for each K step:
+// the K step covers taps 2c and 2c+1; the lane group (lane/16) picks the tap parity and channel half.
+// -1 marks the unused second half of the last step.
+__device__ __forceinline__ void tap_offsets(int (&off)[KC]) {
+  const int kq = 8 * ((threadIdx.x & 63) >> 4);
+#pragma unroll
+  for (int c = 0; c < KC; c++) {
+    const int kb = c * 32 + kq;
+    const int tap = kb / CH, half = (kb % CH) >> 3;
+    off[c] = tap < 27 ? half * kHalfP + (((tap / 9) * S + (tap / 3) % 3) * S + tap % 3) * 8 : -1;
+  }
+}
+
 // One convolution layer from LDS to LDS (MODE 0/1) or to the FP32 output D in LDS (MODE 2).
 template <int MODE>
-__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KC], float b, float* D) {
+__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KC], float b, float* D,
+                                         const int (&off)[KC]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int co = lane & 15, kq = 8 * (lane >> 4);
+  const int co = lane & 15, g = lane >> 4;
   for (int t = wave; t < kCnnVox / 16; t += kThreads / 64) {
     const int v = t * 16 + (lane & 15);
-    const int base = ((v >> 6) * S + ((v >> 3) & 7)) * S + (v & 7);   // padded index of tap (0,0,0)
+    const u16* src = in + (((v >> 6) * S + ((v >> 3) & 7)) * S + (v & 7)) * 8;   // tap (0,0,0), half 0
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < KC; c++) {
-      const int kb = c * 32 + kq;
-      const int tap = kb / CH, half = (kb % CH) >> 3;
       Pack8 a;
-      if (tap < 27) {
-        const int off = ((tap / 9) * S + (tap / 3) % 3) * S + tap % 3;
-        a.u = *reinterpret_cast<const uint4*>(in + half * kHalf + (base + off) * 8);
+      if (c < KC - 1) {
+        a.u = *reinterpret_cast<const uint4*>(src + off[c]);
       } else {
-        a.u = make_uint4(0, 0, 0, 0);
+        const bool live = off[c] >= 0;
+        a.u = *reinterpret_cast<const uint4*>(src + (live ? off[c] : 0));
+        if (!live) a.u = make_uint4(0, 0, 0, 0);
       }
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, wreg[c], acc, 0, 0, 0);
     }
+    // rows 4g..4g+3 of the tile: 4 consecutive voxels along x
+    const int vo0 = t * 16 + 4 * g;
+    if (MODE == 2) {
+      if (co < kCnnOut) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int vo = t * 16 + 4 * (lane >> 4) + r;
-      const float h = acc[r] + b;
-      if (MODE == 2) {
-        if (co < kCnnOut) D[vo * kCnnOut + co] = h;
-      } else {
-        const int q = (((vo >> 6) + 1) * S + ((vo >> 3) & 7) + 1) * S + (vo & 7) + 1;
-        const int c = (co >> 3) * kHalf + q * 8 + (co & 7);
-        float o = fmaxf(h, 0.0f);
-        if (MODE == 1) o += bf2f(in[c]);
-        out[c] = f2bf(o);
+        for (int r = 0; r < 4; r++) D[(vo0 + r) * kCnnOut + co] = acc[r] + b;
+      }
+    } else {
+      const int q0 = (((vo0 >> 6) + 1) * S + ((vo0 >> 3) & 7) + 1) * S + (vo0 & 7) + 1;
+      const int c0 = (co >> 3) * kHalfP + q0 * 8 + (co & 7);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        float o = fmaxf(acc[r] + b, 0.0f);
+        if (MODE == 1) o += bf2f(in[c0 + 8 * r]);
+        out[c0 + 8 * r] = f2bf(o);
       }
     }
   }
@@ -518,9 +532,10 @@ __device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (
 // Zero the border voxels (any coordinate 0 or 9) of both half-planes of a padded activation buffer.
 __device__ __forceinline__ void zero_border(uint4* buf) {
   for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kThreads) {
-    const int vox = i % kCnnPadVox;
+    const int h = i / kCnnPadVox, vox = i % kCnnPadVox;
     const int x = vox % S, y = (vox / S) % S, z = vox / (S * S);
-    if (x == 0 || x == S - 1 || y == 0 || y == S - 1 || z == 0 || z == S - 1) buf[i] = make_uint4(0, 0, 0, 0);
+    if (x == 0 || x == S - 1 || y == 0 || y == S - 1 || z == 0 || z == S - 1)
+      buf[h * (kHalfP / 8) + vox] = make_uint4(0, 0, 0, 0);
   }
 }
 
@@ -554,7 +569,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   constexpr Geo g = geo_of(GS);
   constexpr int N = g.N, R = g.R, C = g.C, OFF = 5 - GS;
   constexpr int R3 = R * R * R, C3 = C * C * C;
-  static_assert(N * N * N * 4 <= kCnnPadVox * CH * 2, "interpolated cube must fit one activation buffer");
+  static_assert(N * N * N * 4 <= kBufVec * 16 && kCnnOutputs * 4 <= kBufVec * 16, "scratch must fit a buffer");
   static_assert(C3 <= kCoarseMax && R3 % 4 == 0 && C3 % 4 == 0, "float4 staging");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float* cube = reinterpret_cast<float*>(L.B);
@@ -585,14 +600,16 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   const float inv = 1.0f / s;
   // packed input, pass 1: interpolated potential (low halves), zero density (high halves)
   uint32_t* A32 = reinterpret_cast<uint32_t*>(A);
-  for (int t = threadIdx.x; t < kCnnPadVox * 8; t += kThreads) {
-    const int vox = t >> 3, d = t & 7;
+  for (int vox = threadIdx.x; vox < kCnnPadVox; vox += kThreads) {
     const int cx = vox % S, cy = (vox / S) % S, cz = vox / (S * S);
-    const int X = 2 * cx - 1 + (d & 1) - OFF, Y = 2 * cy - 1 + ((d >> 1) & 1) - OFF, Z = 2 * cz - 1 + (d >> 2) - OFF;
-    float c0 = 0.0f;
-    if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) c0 = cube[(Z * N + Y) * N + X] * inv;
-    // channels 2d, 2d+1 live in half-plane d/4 at in-half channels 2(d%4), 2(d%4)+1
-    A32[((d >> 2) * kCnnPadVox + vox) * 4 + (d & 3)] = (uint32_t)f2bf(c0);
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+      const int X = 2 * cx - 1 + (d & 1) - OFF, Y = 2 * cy - 1 + ((d >> 1) & 1) - OFF, Z = 2 * cz - 1 + (d >> 2) - OFF;
+      float c0 = 0.0f;
+      if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) c0 = cube[(Z * N + Y) * N + X] * inv;
+      // channels 2d, 2d+1 live in half-plane d/4 at in-half channels 2(d%4), 2(d%4)+1
+      A32[(d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)] = (uint32_t)f2bf(c0);
+    }
   }
   __syncthreads();
   // pass 2: Const*rho/s into the high halves (interior cells only), from the density read above
@@ -608,30 +625,32 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
         const int ex = idx % R + 2 + OFF, ey = (idx / R) % R + 2 + OFF, ez = idx / (R * R) + 2 + OFF;
         const int vox = ((ez >> 1) * S + (ey >> 1)) * S + (ex >> 1);
         const int d = ((ez & 1) * 2 + (ey & 1)) * 2 + (ex & 1);
-        A[(((d >> 2) * kCnnPadVox + vox) * 4 + (d & 3)) * 2 + 1] = f2bf(cst * e4[e] * inv);
+        A[((d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)) * 2 + 1] = f2bf(cst * e4[e] * inv);
       }
     }
   }
-  for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kThreads) L.B[i] = make_uint4(0, 0, 0, 0);
+  for (int i = threadIdx.x; i < kBufVec; i += kThreads) L.B[i] = make_uint4(0, 0, 0, 0);
+  int off[KC];
+  tap_offsets(off);
   __syncthreads();
   load_layer(Wb, P, 1, wb, bb);
-  conv_lds<0>(A, B, wa, ba, nullptr);          // layer 1: A -> B
+  conv_lds<0>(A, B, wa, ba, nullptr, off);     // layer 1: A -> B
   __syncthreads();
   zero_border(L.A);                            // A held the input's border
   load_layer(Wb, P, 2, wa, ba);
-  conv_lds<1>(B, A, wb, bb, nullptr);          // layer 2: B -> A
+  conv_lds<1>(B, A, wb, bb, nullptr, off);     // layer 2: B -> A
   __syncthreads();
   load_layer(Wb, P, 3, wb, bb);
-  conv_lds<1>(A, B, wa, ba, nullptr);          // layer 3
+  conv_lds<1>(A, B, wa, ba, nullptr, off);     // layer 3
   __syncthreads();
   load_layer(Wb, P, 4, wa, ba);
-  conv_lds<1>(B, A, wb, bb, nullptr);          // layer 4
+  conv_lds<1>(B, A, wb, bb, nullptr, off);     // layer 4
   __syncthreads();
   load_layer(Wb, P, 5, wb, bb);
-  conv_lds<1>(A, B, wa, ba, nullptr);          // layer 5
+  conv_lds<1>(A, B, wa, ba, nullptr, off);     // layer 5
   __syncthreads();
   float* D = reinterpret_cast<float*>(L.A);
-  conv_lds<2>(B, nullptr, wb, bb, D);          // layer 6: B -> D (in A)
+  conv_lds<2>(B, nullptr, wb, bb, D, off);     // layer 6: B -> D (in A)
   __syncthreads();
   interp_cube_lds<GS>(L.cs, int_scheme, cube);  // B is free again
   for (int t = threadIdx.x; t < R3; t += kThreads) {

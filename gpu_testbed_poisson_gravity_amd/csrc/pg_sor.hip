@@ -74,32 +74,46 @@ __device__ __forceinline__ bool wave_any(bool pred) { return __ballot(pred) != 0
 // ---- interpolation into LDS --------------------------------------------------------------------------
 
 // Interpolate a patch's coarse potential into its LDS cube (split layout).  `lane`/`nlanes` index the
-// lanes serving this patch.
+// lanes serving this patch.  Two coarse cells per lane and step: both 27-value neighbourhoods are
+// requested before either is interpolated, halving the memory round trips of the phase.
 template <int GS>
 __device__ void interp_to_lds(const float* __restrict__ cp, float* __restrict__ lds, int lane, int nlanes,
                               int int_scheme) {
   using S = SorGeo<GS>;
   constexpr int C = S::C, N = S::N, U = S::U, HP = S::HP, NC = C - 2;
-  for (int q = lane; q < NC * NC * NC; q += nlanes) {
-    const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
-    float v[3][3][3];
+  constexpr int NCELL = NC * NC * NC;
+  for (int q0 = lane; q0 < NCELL; q0 += 2 * nlanes) {
+    float v[2][3][3][3];
 #pragma unroll
-    for (int dk = 0; dk < 3; dk++)
+    for (int u = 0; u < 2; u++) {
+      const int q = q0 + u * nlanes < NCELL ? q0 + u * nlanes : q0;
+      const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
 #pragma unroll
-      for (int dj = 0; dj < 3; dj++)
+      for (int dk = 0; dk < 3; dk++)
 #pragma unroll
-        for (int di = 0; di < 3; di++) v[dk][dj][di] = cp[((k + dk - 1) * C + (j + dj - 1)) * C + (i + di - 1)];
-    float ch[2][2][2];
-    interp_children(v, int_scheme, ch);
+        for (int dj = 0; dj < 3; dj++)
 #pragma unroll
-    for (int cz = 0; cz < 2; cz++)
+          for (int di = 0; di < 3; di++)
+            v[u][dk][dj][di] = cp[((k + dk - 1) * C + (j + dj - 1)) * C + (i + di - 1)];
+    }
 #pragma unroll
-      for (int cy = 0; cy < 2; cy++)
+    for (int u = 0; u < 2; u++) {
+      const int q = q0 + u * nlanes;
+      if (q >= NCELL) break;
+      const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
+      float ch[2][2][2];
+      interp_children(v[u], int_scheme, ch);
 #pragma unroll
-        for (int cx = 0; cx < 2; cx++) {
-          const int x = 2 * (i - 1) + cx - U, y = 2 * (j - 1) + cy - U, z = 2 * (k - 1) + cz - U;
-          if (x >= 0 && x < N && y >= 0 && y < N && z >= 0 && z < N) lds[cube_index(N, HP, x, y, z)] = ch[cz][cy][cx];
-        }
+      for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+        for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+          for (int cx = 0; cx < 2; cx++) {
+            const int x = 2 * (i - 1) + cx - U, y = 2 * (j - 1) + cy - U, z = 2 * (k - 1) + cz - U;
+            if (x >= 0 && x < N && y >= 0 && y < N && z >= 0 && z < N)
+              lds[cube_index(N, HP, x, y, z)] = ch[cz][cy][cx];
+          }
+    }
   }
 }
 
@@ -231,36 +245,52 @@ struct LanePatch {
 };
 
 // Gravity update of one patch from the potential held in an LDS cube (CPU_GravitySolver.cpp:34-79).
-template <int GS>
-__device__ void gravity_from_lds(const float* __restrict__ lds, float* __restrict__ flu, int lane, int nlanes,
-                                 float gra_const, int p5) {
+// The SEG lanes serving the patch take cells lane, lane+SEG, ...; the fluid of up to 8 cells per lane is
+// requested before any of them is updated, so a lane waits for memory once per chunk, not once per cell.
+template <int GS, int SEG>
+__device__ void gravity_from_lds(const float* __restrict__ lds, float* __restrict__ flu, int lane, float gra_const,
+                                 int p5) {
   using S = SorGeo<GS>;
   constexpr int N = S::N, HP = S::HP;
-  for (int t = lane; t < kPatchCells; t += nlanes) {
-    const int i = t % kPatchSize, j = (t / kPatchSize) % kPatchSize, k = t / (kPatchSize * kPatchSize);
-    const int x = i + S::gs, y = j + S::gs, z = k + S::gs;
-    auto P = [&](int dz, int dy, int dx) { return lds[cube_index(N, HP, x + dx, y + dy, z + dz)]; };
-    float ax, ay, az;
-    if (p5) {
-      ax = gra_const * (-P(0, 0, 2) + P(0, 0, -2) + 8.0f * P(0, 0, 1) - 8.0f * P(0, 0, -1));
-      ay = gra_const * (-P(0, 2, 0) + P(0, -2, 0) + 8.0f * P(0, 1, 0) - 8.0f * P(0, -1, 0));
-      az = gra_const * (-P(2, 0, 0) + P(-2, 0, 0) + 8.0f * P(1, 0, 0) - 8.0f * P(-1, 0, 0));
-    } else {
-      ax = gra_const * (P(0, 0, 1) - P(0, 0, -1));
-      ay = gra_const * (P(0, 1, 0) - P(0, -1, 0));
-      az = gra_const * (P(1, 0, 0) - P(-1, 0, 0));
+  constexpr int NT = kPatchCells / SEG;          // cells per lane
+  constexpr int CHUNK = NT < 8 ? NT : 8;
+  static_assert(kPatchCells % SEG == 0 && NT % CHUNK == 0, "lane segment must divide the patch");
+  for (int c0 = 0; c0 < NT; c0 += CHUNK) {
+    float f[CHUNK][kNComp];
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      const int t = lane + (c0 + u) * SEG;
+#pragma unroll
+      for (int v = 0; v < kNComp; v++) f[u][v] = flu[v * kPatchCells + t];
     }
-    const float rho = flu[t];
-    float px = flu[kPatchCells + t], py = flu[2 * kPatchCells + t], pz = flu[3 * kPatchCells + t];
-    const float temp = 0.5f / rho;
-    const float eint = flu[4 * kPatchCells + t] - temp * (px * px + py * py + pz * pz);
-    px = px - rho * ax;
-    py = py - rho * ay;
-    pz = pz - rho * az;
-    flu[kPatchCells + t] = px;
-    flu[2 * kPatchCells + t] = py;
-    flu[3 * kPatchCells + t] = pz;
-    flu[4 * kPatchCells + t] = eint + temp * (px * px + py * py + pz * pz);
+#pragma unroll
+    for (int u = 0; u < CHUNK; u++) {
+      const int t = lane + (c0 + u) * SEG;
+      const int i = t % kPatchSize, j = (t / kPatchSize) % kPatchSize, k = t / (kPatchSize * kPatchSize);
+      const int x = i + S::gs, y = j + S::gs, z = k + S::gs;
+      auto P = [&](int dz, int dy, int dx) { return lds[cube_index(N, HP, x + dx, y + dy, z + dz)]; };
+      float ax, ay, az;
+      if (p5) {
+        ax = gra_const * (-P(0, 0, 2) + P(0, 0, -2) + 8.0f * P(0, 0, 1) - 8.0f * P(0, 0, -1));
+        ay = gra_const * (-P(0, 2, 0) + P(0, -2, 0) + 8.0f * P(0, 1, 0) - 8.0f * P(0, -1, 0));
+        az = gra_const * (-P(2, 0, 0) + P(-2, 0, 0) + 8.0f * P(1, 0, 0) - 8.0f * P(-1, 0, 0));
+      } else {
+        ax = gra_const * (P(0, 0, 1) - P(0, 0, -1));
+        ay = gra_const * (P(0, 1, 0) - P(0, -1, 0));
+        az = gra_const * (P(1, 0, 0) - P(-1, 0, 0));
+      }
+      const float rho = f[u][0];
+      float px = f[u][1], py = f[u][2], pz = f[u][3];
+      const float temp = 0.5f / rho;
+      const float eint = f[u][4] - temp * (px * px + py * py + pz * pz);
+      px = px - rho * ax;
+      py = py - rho * ay;
+      pz = pz - rho * az;
+      flu[kPatchCells + t] = px;
+      flu[2 * kPatchCells + t] = py;
+      flu[3 * kPatchCells + t] = pz;
+      flu[4 * kPatchCells + t] = eint + temp * (px * px + py * py + pz * pz);
+    }
   }
 }
 
@@ -423,7 +453,7 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
       const int i = t % G, j = (t / G) % G, k = t / (G * G);
       lp.out[t] = lds[cube_index(N, HP, i + o, j + o, k + o)];
     }
-    if (P.fuse_gravity && lp.flu) gravity_from_lds<GS>(lds, lp.flu, pl, SEG, P.gra_const, P.p5);
+    if (P.fuse_gravity && lp.flu) gravity_from_lds<GS, SEG>(lds, lp.flu, pl, P.gra_const, P.p5);
   }
   __syncthreads();
 }
