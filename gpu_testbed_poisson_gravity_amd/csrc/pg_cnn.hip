@@ -4,7 +4,9 @@
 //
 // Network (activations BF16 channel-last, weights BF16, accumulation and master weights FP32):
 //   frame   the 18^3 solve cube of ghost size 5 (smaller cubes centred, zeros around), with the fields
-//           q0 = interp/s and q1 = Poi_Coeff*dh^2*rho/s (solve interior only), s = max|interp|
+//           q0 = interp/s and q1 = 64 r/s (solve interior only), s = max|interp|, r the SOR residual of
+//           the interpolated cube (reference operand order), i.e. the right-hand side of A e = r that the
+//           correction e to the interpolated start satisfies
 //   x0      [n][10^3][16]: space-to-depth of the zero-extended 20^3 frame (fine index f = 2c - 1 + d per
 //           axis, channel ((dz*2+dy)*2+dx)*2 + q), so that coarse voxels 1..8 hold the 2^3 blocks of the
 //           16^3 interior and voxels 0 and 9 hold the Dirichlet layer
@@ -329,6 +331,15 @@ __global__ void weight_copies_kernel(const float* __restrict__ P, u16* __restric
 
 // ---- input preparation and guess assembly -------------------------------------------------------------
 
+constexpr float kResScale = 64.0f;   // q1 = kResScale * r / s
+
+// SOR residual of the interpolated cube at interior cube cell (X, Y, Z) (1..R), reference operand order.
+template <int N>
+__device__ __forceinline__ float interp_residual(const float* cube, int X, int Y, int Z, float crho) {
+  const float* c = cube + (Z * N + Y) * N + X;
+  return c[N * N] + c[-N * N] + c[N] + c[-N] + c[1] + c[-1] - 6.0f * c[0] - crho;
+}
+
 // Interpolated solve cube of one patch into LDS (N^3 floats).
 template <int GS>
 __device__ void interp_cube(const float* __restrict__ cpot, int int_scheme, float* cube) {
@@ -382,7 +393,7 @@ __device__ void prep_patch(const float* __restrict__ rho, const float* __restric
     if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) {
       c0 = cube[(Z * N + Y) * N + X] * inv;
       if (X >= 1 && X <= R && Y >= 1 && Y <= R && Z >= 1 && Z <= R)
-        c1 = cst * rho[((Z - 1) * R + (Y - 1)) * R + (X - 1)] * inv;
+        c1 = (interp_residual<N>(cube, X, Y, Z, cst * rho[((Z - 1) * R + (Y - 1)) * R + (X - 1)]) * inv) * kResScale;
     }
     reinterpret_cast<uint32_t*>(x)[t] = (uint32_t)f2bf(c0) | ((uint32_t)f2bf(c1) << 16);
   }
@@ -455,9 +466,9 @@ __global__ void __launch_bounds__(kThreads) guess_kernel(const float* __restrict
 //   for the final interpolation, the density is read with float4 loads and scattered into the packed
 //   input, and each layer's weights are loaded into registers while the previous layer computes.
 
-// Half-plane stride: 1000 voxels x 8 channels plus 64 bytes, so that channels 0-7 and 8-15 of a voxel
-// fall in different banks (the 2-byte output writes of the two channel halves do not collide).
-constexpr int kHalfP = kCnnPadVox * 8 + 32;  // bf16 values per half-plane (with padding)
+// Half-plane stride (bf16 values).  Unpadded: measured on MI355X, padding the stride by 32, 64, 128 or
+// 256 bytes made the kernel 2-13% slower (more LDS bank-conflict cycles on the 16-byte A-operand reads).
+constexpr int kHalfP = kCnnPadVox * 8;
 constexpr int kBufVec = kHalfP * 2 / 8;      // uint4 per activation buffer
 constexpr int kCoarseMax = 12 * 12 * 12;     // C^3 at ghost size 5
 
@@ -612,7 +623,8 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
     }
   }
   __syncthreads();
-  // pass 2: Const*rho/s into the high halves (interior cells only), from the density read above
+  // pass 2: 64 r/s into the high halves (interior cells only; r from the cube, still in B, and the density
+  // read above)
 #pragma unroll
   for (int k = 0; k < RV; k++) {
     const int i4 = threadIdx.x + k * kThreads;
@@ -622,13 +634,16 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
       for (int e = 0; e < 4; e++) {
         const int idx = 4 * i4 + e;
         // interior cell -> extended frame index 2 + local + OFF -> coarse voxel (>>1), sub-cell (&1)
-        const int ex = idx % R + 2 + OFF, ey = (idx / R) % R + 2 + OFF, ez = idx / (R * R) + 2 + OFF;
+        const int lx = idx % R, ly = (idx / R) % R, lz = idx / (R * R);
+        const int ex = lx + 2 + OFF, ey = ly + 2 + OFF, ez = lz + 2 + OFF;
         const int vox = ((ez >> 1) * S + (ey >> 1)) * S + (ex >> 1);
         const int d = ((ez & 1) * 2 + (ey & 1)) * 2 + (ex & 1);
-        A[((d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)) * 2 + 1] = f2bf(cst * e4[e] * inv);
+        const float r = interp_residual<N>(cube, lx + 1, ly + 1, lz + 1, cst * e4[e]);
+        A[((d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)) * 2 + 1] = f2bf((r * inv) * kResScale);
       }
     }
   }
+  __syncthreads();   // every lane is done reading the cube in B
   for (int i = threadIdx.x; i < kBufVec; i += kThreads) L.B[i] = make_uint4(0, 0, 0, 0);
   int off[KC];
   tap_offsets(off);

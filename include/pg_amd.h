@@ -182,7 +182,8 @@ int pgamd_graph_destroy(pgamd_graph* graph);
  * A 3D CNN (BF16 weights, FP32 accumulation, hand-written MFMA convolutions) predicts the correction from
  * the interpolated coarse potential to the converged potential on the whole solve interior.  Frame: the
  * 18^3 solve cube of ghost size 5 (smaller cubes centred, zeros around) with q0 = interp/s and
- * q1 = Poi_Coeff*dh^2*rho/s (interior only), s = max|interp| of the patch.  Input x0 [n][10^3][16] BF16:
+ * q1 = 64*r/s (interior only), r the SOR residual of the interpolated cube (reference operand order),
+ * s = max|interp| of the patch.  Input x0 [n][10^3][16] BF16:
  * space-to-depth of the zero-extended 20^3 frame (fine index f = 2c-1+d per axis, channel
  * ((dz*2+dy)*2+dx)*2 + q).  Six 3^3 convolutions with 16 channels on the 8^3 coarse grid (the first
  * valid on the 10^3 input, the next four residual blocks y += relu(conv(y)+b), the last linear with 8

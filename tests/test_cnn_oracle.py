@@ -77,11 +77,16 @@ def test_packing_index_arithmetic():
 
 
 def test_frame_centres_smaller_cubes():
-    cube = np.arange(10 ** 3, dtype=np.float32).reshape(10, 10, 10) + 1
+    # q1 is 64 r / s with r the 7-point residual of the cube: on phi = x^2 + y^2 + z^2 (cell units) the
+    # discrete Laplacian is exactly 6, so r = 6 - Const*rho at every interior cell
+    ax = np.arange(10, dtype=np.float32)
+    cube = (ax[:, None, None] ** 2 + ax[None, :, None] ** 2 + ax[None, None, :] ** 2).astype(np.float32)
     rho = np.ones((8, 8, 8), np.float32)
     q0, q1 = K.frame18(cube, rho, gs=1, cst=2.0, scale=4.0)
-    assert q0[4, 4, 4] == cube[0, 0, 0] / 4 and q0[3].max() == 0 and q0[14].max() == 0
-    assert q1[4].max() == 0 and q1[5, 5, 5] == 0.5 and q1[13, 12, 12] == 0
+    assert q0[4, 4, 4] == cube[0, 0, 0] / 4 and q0[5, 4, 4] == cube[1, 0, 0] / 4
+    assert q0[3].max() == 0 and q0[14].max() == 0                # gs 1: the 10^3 cube sits at 4..13
+    assert np.all(q1[5:13, 5:13, 5:13] == 64.0 * (6.0 - 2.0) / 4.0)
+    assert q1[4].max() == 0 and q1[13].max() == 0 and q1[:, 4].max() == 0   # Dirichlet layer and outside
 
 
 def test_bf16_round_ties_to_even():
