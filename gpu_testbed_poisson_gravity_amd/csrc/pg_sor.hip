@@ -314,29 +314,38 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
   __syncthreads();
 
   // B. owned cells into registers
+  // The lane's two x-neighbours (xx = 0, 1) are adjacent in memory: one 8-byte load reads both, so every
+  // load instruction covers whole cache lines (no line is requested twice).
   LaneCells<GS> L;
 #pragma unroll
-  for (int yy = 0; yy < 2; yy++)
+  for (int yy = 0; yy < 2; yy++) {
 #pragma unroll
     for (int xx = 0; xx < 2; xx++) {
       L.bz[yy][xx][0] = lds[lb + own_off<GS>((xx + yy) & 1, 0, yy, xx)];
       L.bz[yy][xx][1] = lds[lb + own_off<GS>((xx + yy + N - 1) & 1, N - 1, yy, xx)];
+    }
 #pragma unroll
-      for (int zi = 0; zi < R; zi++) {
+    for (int zi = 0; zi < R; zi++) {
+      const int gi = (zi * R + 2 * yb + yy) * R + 2 * m;
+      const float2 r2 = active ? *reinterpret_cast<const float2*>(lp.rho + gi) : make_float2(0.0f, 0.0f);
+      float2 g2 = make_float2(0.0f, 0.0f);
+      if (P.init_mode == 2 && active) g2 = *reinterpret_cast<const float2*>(lp.guess + gi);
+#pragma unroll
+      for (int xx = 0; xx < 2; xx++) {
         const int c = (xx + yy + zi + 1) & 1;
-        const int gi = (zi * R + 2 * yb + yy) * R + 2 * m + xx;
         float v;
         if (P.init_mode == 0)
           v = lds[lb + own_off<GS>(c, zi + 1, yy, xx)];
         else if (P.init_mode == 1)
           v = 0.0f;
         else
-          v = active ? lp.guess[gi] : 0.0f;
+          v = xx ? g2.y : g2.x;
         L.phi[yy][xx][zi] = v;
-        L.cr[yy][xx][zi] = active ? P.cst * lp.rho[gi] : 0.0f;
+        L.cr[yy][xx][zi] = active ? P.cst * (xx ? r2.y : r2.x) : 0.0f;
         if (P.init_mode != 0 && active) lds[lb + own_off<GS>(c, zi + 1, yy, xx)] = v;
       }
     }
+  }
   __syncthreads();
 
   // B'. reject a caller guess whose relative residual exceeds guess_tol: restart from the zero guess
