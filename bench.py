@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--fallback-tol", type=float, default=1e-2,
                     help="guess check: a network guess with relative residual above this restarts from zero")
     ap.add_argument("--mode", choices=["both", "plain", "warm"], default="both")
+    ap.add_argument("--chunks", type=int, default=0,
+                    help="also time the pipelined warm start (pgamd_cnn_solve_device) over this many slices; "
+                         "0: serial only (measured faster on MI355X: the two kernels do not share CUs well)")
     ap.add_argument("--workload", choices=["config2", "config3", "config5"], default="config2",
                     help="config2 (default): the metric's workload; config3: 32768 mixed-GS lognormal patches, "
                          "half SOR half MG, one graph-captured persistent launch; config5: 65536 mixed-GS "
@@ -215,8 +218,9 @@ def main():
         torch.cuda.synchronize(dev)
         del src, dst
 
-    def run(init_mode, net):
-        """warmup + timed steps; events bracket the network phase and the solve separately."""
+    def run(init_mode, net, pipelined=False):
+        """warmup + timed steps; events bracket the network phase and the solve separately (serial legs)
+        or the whole pipelined warm-start solve."""
         ctx.set_options(pot_ghost_size=a.gs, gra_ghost_size=1, pot_scheme=pg.SOR, init_mode=init_mode,
                         term_mode=pg.TERM_RELRES, check_every=a.check_every, rel_tol=1e-5, fuse_gravity=1,
                         guess_fallback_tol=a.fallback_tol if net is not None else 0.0)
@@ -225,12 +229,18 @@ def main():
             d_flu.copy_(d_flu0)  # restore the fluid so every step does identical work (D2D, outside events)
             if e is not None:
                 e[0].record(stream)
-            if net is not None:
-                net.warm_start(d_rho, d_cpot, a.npatch, args, d_guess, stream=sp)
-            if e is not None:
-                e[1].record(stream)
-            ctx.solve_device(d_rho, d_cpot, d_out, d_flu, a.npatch, args, guess=d_guess if net is not None else None,
-                             iters=d_it, status=d_st, stream=sp)
+            if pipelined:
+                if e is not None:
+                    e[1].record(stream)
+                net.solve_device(d_rho, d_cpot, d_guess, d_out, d_flu, a.npatch, args, iters=d_it, status=d_st,
+                                 nchunk=a.chunks, stream=sp)
+            else:
+                if net is not None:
+                    net.warm_start(d_rho, d_cpot, a.npatch, args, d_guess, stream=sp)
+                if e is not None:
+                    e[1].record(stream)
+                ctx.solve_device(d_rho, d_cpot, d_out, d_flu, a.npatch, args,
+                                 guess=d_guess if net is not None else None, iters=d_it, status=d_st, stream=sp)
             if e is not None:
                 e[2].record(stream)
 
@@ -261,7 +271,7 @@ def main():
                 "algorithmic_bytes_per_launch": alg, "launch_ms": r["solve_ms"],
                 "hbm_achieved": hbm / (r["solve_ms"] * 1e-3) / 1e9, "hbm_peak": HBM_PEAK / 1e9}
 
-    plain = warm = None
+    plain = warm = warm_serial = None
     if a.mode in ("both", "plain"):
         plain = run(pg.INIT_ZERO, None)
         log(f"[bench] plain: {plain['value']:.4g} patch-solves/s, {plain['mean_iters']:.1f} iterations")
@@ -270,23 +280,32 @@ def main():
         if os.path.exists(a.weights):
             net = pg.CNN(ctx)
             net.load_params(a.weights)
-            warm = run(pg.INIT_GUESS, net)
-            log(f"[bench] warm: {warm['value']:.4g} patch-solves/s, {warm['mean_iters']:.1f} iterations, "
-                f"network {warm['network_ms']:.3f} ms + solve {warm['solve_ms']:.3f} ms")
+            warm_serial = run(pg.INIT_GUESS, net)
+            log(f"[bench] warm (serial): {warm_serial['value']:.4g} patch-solves/s, "
+                f"{warm_serial['mean_iters']:.1f} iterations, network {warm_serial['network_ms']:.3f} ms + "
+                f"solve {warm_serial['solve_ms']:.3f} ms")
+            warm = warm_serial
+            if a.chunks > 0:
+                warm_pipe = run(pg.INIT_GUESS, net, pipelined=True)
+                log(f"[bench] warm (pipelined, {a.chunks} slices): {warm_pipe['value']:.4g} patch-solves/s, "
+                    f"{warm_pipe['mean_iters']:.1f} iterations")
+                if warm_pipe["value"] > warm["value"]:
+                    warm = warm_pipe
         else:
             log(f"[bench] no trained weights at {a.weights}: warm-start leg skipped")
     head_r = warm if warm is not None else plain
-    roof_sor = lds_roof(head_r)
+    # per-kernel rooflines come from the serial legs (each kernel timed alone on the stream)
+    roof_sor = lds_roof(warm_serial if warm_serial is not None else plain)
     roofs = {"sor": roof_sor}
     roof = roof_sor
-    if warm is not None:
+    if warm_serial is not None:
         flop = CNN_MFMA_FLOP * a.npatch
-        ach = flop / (warm["network_ms"] * 1e-3)
+        ach = flop / (warm_serial["network_ms"] * 1e-3)
         roofs["cnn"] = {"kernel": "fused_guess_kernel (interpolation, packing, 6 MFMA convolutions in LDS, guess)",
                         "bound": "mfma", "achieved": ach / 1e12, "peak": MFMA_BF16_PEAK / 1e12, "unit": "TFLOP/s",
                         "frac": ach / MFMA_BF16_PEAK, "traffic": None, "algorithmic_flop_per_launch": flop,
-                        "launch_ms": warm["network_ms"]}
-        if warm["network_ms"] > warm["solve_ms"]:
+                        "launch_ms": warm_serial["network_ms"]}
+        if warm_serial["network_ms"] > warm_serial["solve_ms"]:
             roof = roofs["cnn"]
 
     cpu = None
@@ -313,8 +332,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{a.npatch} patches gs={a.gs} (18^3 cube), Plummer 1-8 spheres, SOR omega=1.8, "
                                f"rel. residual 1e-5, min 10 / max 1000 iterations, fused gravity; value = "
-                               + ("learned warm start (guess check 1e-2)" if warm is not None else
-                                  "zero guess (no trained weights)"),
+                               + ("learned warm start (network, then SOR from its guess with the guess check at 1e-2)"
+                                  if warm is not None else "zero guess (no trained weights)"),
                    "npatch": a.npatch, "gs": a.gs, "check_every": a.check_every},
         "plain": strip(plain), "warm": strip(warm),
         "warm_vs_plain": (warm["value"] / plain["value"]) if (warm is not None and plain is not None) else None,

@@ -121,6 +121,9 @@ SIGNATURES = {
                                        C.c_void_p, C.c_void_p]),
     "pgamd_cnn_warm_start_mixed": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.POINTER(SolverArgs), C.c_void_p, C.c_void_p]),
+    "pgamd_cnn_solve_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_int, C.POINTER(SolverArgs), C.c_void_p, C.c_void_p,
+                                         C.c_int, C.c_void_p]),
 }
 
 _lib = None
@@ -440,3 +443,10 @@ class CNN:
     def warm_start_mixed(self, plan, rho, cpot, args: SolverArgs, guess, stream=None):
         _check(load().pgamd_cnn_warm_start_mixed(self.ctx.h, self.h, plan.h, _vp(rho), _vp(cpot), C.byref(args),
                                                  _vp(guess), stream))
+
+    def solve_device(self, rho, cpot, guess, pot_out, flu, npatch, args: SolverArgs, iters=None, status=None,
+                     nchunk=4, stream=None):
+        """Warm start + solve, pipelined over nchunk slices (context init_mode must be INIT_GUESS)."""
+        _check(load().pgamd_cnn_solve_device(self.ctx.h, self.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out),
+                                             _vp(flu), npatch, C.byref(args), _vp(iters), _vp(status), nchunk,
+                                             stream))

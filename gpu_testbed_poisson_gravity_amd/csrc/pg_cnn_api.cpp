@@ -27,6 +27,7 @@ namespace {
 constexpr int kW = kCnnChannels * kCnnK;   // weights per layer
 constexpr int kMseBlocks = 64;
 constexpr int kMaxReduceBlocks = 4096;
+constexpr int kMaxChunks = 16;               // pgamd_cnn_solve_device
 
 int ow(int l) { return (l - 1) * kCnnLayerParams; }   // layer l = 1..6
 int ob(int l) { return ow(l) + kW; }
@@ -47,6 +48,8 @@ struct pgamd_cnn {
   uint16_t* dh = nullptr;   // [n][8^3][16]
   float *d = nullptr, *dy6 = nullptr, *dyA = nullptr, *dyB = nullptr, *part = nullptr, *lossp = nullptr;
   size_t part_floats = 0;
+  // pipelined warm-start solve: fork event and one event per chunk
+  hipEvent_t ev[kMaxChunks + 1] = {};
 };
 
 namespace {
@@ -214,6 +217,8 @@ int pgamd_cnn_destroy(pgamd_cnn* n) {
   hipFree(n->v);
   hipFree(n->wb);
   hipFree(n->wt);
+  for (hipEvent_t e : n->ev)
+    if (e) hipEventDestroy(e);
   delete n;
   return PGAMD_OK;
 }
@@ -322,6 +327,49 @@ int pgamd_cnn_warm_start_mixed(pgamd_context* ctx, pgamd_cnn* n, pgamd_plan* pla
   const float cst = a->poi_coeff * a->dh * a->dh;
   CK(cnn_fused_guess_mixed(d_rho, d_pot_in, plan_device_desc(plan), n->wb, n->p, np, cst, a->int_scheme, d_guess,
                            (hipStream_t)stream));
+  return PGAMD_OK;
+}
+
+// Warm start and solve of a uniform batch, pipelined: the batch is cut into nchunk slices; the network of
+// slice i+1 runs on the context's second stream while slice i is solved on `stream`, so the matrix-core
+// network and the VALU-bound SOR sweeps share the GPU instead of running back to back.  Results are those
+// of pgamd_cnn_warm_start followed by pgamd_solve_device (same kernels, same inputs per slice).
+int pgamd_cnn_solve_device(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho, const float* d_pot_in, float* d_guess,
+                           float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* a, int* d_iters,
+                           int* d_status, int nchunk, void* stream) {
+  if (!ctx || !n || !a || !d_rho || !d_pot_in || !d_guess || !d_pot_out)
+    return set_error(PGAMD_ERR_ARG, "null argument");
+  if (npatch < 0 || nchunk < 1 || nchunk > kMaxChunks) return set_error(PGAMD_ERR_ARG, "bad patch or chunk count");
+  const pgamd_options& o = ctx_options(ctx);
+  if (o.init_mode != PGAMD_INIT_GUESS || !a->poisson)
+    return set_error(PGAMD_ERR_ARG, "the warm-start solve needs init_mode GUESS and poisson = 1");
+  if (a->int_scheme != PGAMD_INT_CQUAD && a->int_scheme != PGAMD_INT_QUAD)
+    return set_error(PGAMD_ERR_ARG, "incorrect parameter IntScheme");
+  if (npatch == 0) return PGAMD_OK;
+  if (hipSetDevice(n->device) != hipSuccess) return set_error(PGAMD_ERR_DEVICE, "hipSetDevice failed");
+  const Geo g = geo_of(o.pot_ghost_size);
+  const size_t R3 = (size_t)g.R * g.R * g.R, C3 = (size_t)g.C * g.C * g.C;
+  const size_t G = gra_nxt(o.gra_ghost_size), G3 = G * G * G, F = (size_t)kNComp * kPatchCells;
+  hipStream_t s0 = (hipStream_t)stream, s1 = ctx_aux_stream(ctx);
+  if (!s1) return set_error(PGAMD_ERR_DEVICE, "auxiliary stream creation failed");
+  for (hipEvent_t& e : n->ev)
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return set_error(PGAMD_ERR_DEVICE, "hipEventCreate failed");
+  const float cst = a->poi_coeff * a->dh * a->dh;
+  CK(hipEventRecord(n->ev[kMaxChunks], s0));  // the network waits for the caller's earlier work
+  CK(hipStreamWaitEvent(s1, n->ev[kMaxChunks], 0));
+  const int chunk = (npatch + nchunk - 1) / nchunk;
+  for (int i = 0, off = 0; off < npatch; i++, off += chunk) {
+    const int m = std::min(chunk, npatch - off);
+    CK(cnn_fused_guess(d_rho + off * R3, d_pot_in + off * C3, n->wb, n->p, m, o.pot_ghost_size, cst, a->int_scheme,
+                       d_guess + off * R3, s1));
+    CK(hipEventRecord(n->ev[i], s1));
+    CK(hipStreamWaitEvent(s0, n->ev[i], 0));
+    const int rc = pgamd_solve_device(ctx, d_rho + off * R3, d_pot_in + off * C3, d_guess + off * R3,
+                                      d_pot_out + off * G3, d_flu ? d_flu + off * F : nullptr, m, a,
+                                      d_iters ? d_iters + off : nullptr, d_status ? d_status + off : nullptr, s0);
+    if (rc != PGAMD_OK) return rc;
+  }
   return PGAMD_OK;
 }
 

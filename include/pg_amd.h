@@ -210,6 +210,13 @@ int pgamd_cnn_train_step(pgamd_cnn* net, const uint16_t* d_x0, const float* d_ta
 /* Guess of the interior ([n][R^3]) for a batch at the context's ghost size: prepare, forward, assemble. */
 int pgamd_cnn_warm_start(pgamd_context* ctx, pgamd_cnn* net, const float* d_rho, const float* d_pot_in, int npatch,
                          const pgamd_solver_args* args, float* d_guess, void* stream);
+/* Warm start and solve of a uniform batch in one call, pipelined over nchunk (1..16) slices: the network
+ * of slice i+1 runs on the context's second stream while slice i is solved on `stream`.  The context must
+ * have init_mode PGAMD_INIT_GUESS (guess_fallback_tol applies); d_guess ([n][R^3]) is workspace and
+ * receives the guesses.  Same results as pgamd_cnn_warm_start + pgamd_solve_device. */
+int pgamd_cnn_solve_device(pgamd_context* ctx, pgamd_cnn* net, const float* d_rho, const float* d_pot_in,
+                           float* d_guess, float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* args,
+                           int* d_iters, int* d_status, int nchunk, void* stream);
 /* Guess for a mixed batch (plan): d_rho / d_pot_in packed as the plan says; d_guess is packed like d_rho
  * (patch p at rho_off[p], R(p)^3 values) and goes to pgamd_solve_mixed_device. */
 int pgamd_cnn_warm_start_mixed(pgamd_context* ctx, pgamd_cnn* net, pgamd_plan* plan, const float* d_rho,

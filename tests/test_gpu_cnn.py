@@ -380,3 +380,42 @@ def test_mixed_warm_start_matches_uniform():
         finally:
             plan.close()
             net.close()
+
+
+@pytest.mark.parametrize("nchunk", [1, 3, 16])
+def test_pipelined_warm_solve_matches_serial(nchunk):
+    """pgamd_cnn_solve_device (network of slice i+1 on a second stream while slice i is solved) gives the
+    results of pgamd_cnn_warm_start followed by pgamd_solve_device, bit for bit."""
+    torch = _torch()
+    gs, n = 5, 40
+    rho, cpot, flu, _ = I.plummer(n, gs, seed=70 + nchunk)
+    args = pg.solver_args(dt=1e-3, sor_min_iter=10, sor_max_iter=1000, sor_omega=1.8, poi_coeff=POI)
+    with pg.Context(0) as c:
+        c.set_options(pot_ghost_size=gs, init_mode=pg.INIT_GUESS, term_mode=pg.TERM_RELRES, rel_tol=1e-5,
+                      guess_fallback_tol=1e-2, fuse_gravity=1)
+        net = pg.CNN(c)
+        try:
+            net.set_params(_random_params(4) * 0.05)
+            d_rho, d_cpot = _dev(rho), _dev(cpot)
+            res = []
+            for pipelined in (False, True):
+                guess = torch.zeros((n, 16, 16, 16), device="cuda")
+                out = torch.zeros((n, 10, 10, 10), device="cuda")
+                d_flu = _dev(flu)
+                it = torch.zeros(n, dtype=torch.int32, device="cuda")
+                st = torch.zeros(n, dtype=torch.int32, device="cuda")
+                print(f"[case] warm solve nchunk={nchunk} pipelined={pipelined}", flush=True)
+                if pipelined:
+                    net.solve_device(d_rho, d_cpot, guess, out, d_flu, n, args, iters=it, status=st, nchunk=nchunk)
+                else:
+                    net.warm_start(d_rho, d_cpot, n, args, guess)
+                    c.solve_device(d_rho, d_cpot, out, d_flu, n, args, guess=guess, iters=it, status=st)
+                torch.cuda.synchronize()
+                res.append([x.cpu().numpy() for x in (guess, out, d_flu, it, st)])
+            with pytest.raises(pg.PgError):
+                net.solve_device(d_rho, d_cpot, guess, out, d_flu, n, args, nchunk=17)
+        finally:
+            net.close()
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+    assert np.all((res[1][4] & pg.STATUS_CONVERGED) != 0)
