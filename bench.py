@@ -29,10 +29,27 @@ LDS_BYTES_PER_UPDATE = 36  # 7 phi reads + 1 rho read + 1 write, FP32
 # (the output layer's 8 live channels run on a 16-wide MFMA tile): 6 x 512 x 16 x 432 x 2 on the matrix cores
 CNN_MFMA_FLOP = 6 * 512 * 16 * 432 * 2
 WEIGHTS = os.path.join(ROOT, "gpu_testbed_poisson_gravity_amd", "weights", "warm_start_gs5.npy")
+# HBM bytes per launch of each kernel on the default workload, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
+# (each scaled by what the counter reads for a 1 GiB device copy); lines "traffic <kernel> <leg> <bytes>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_plummer16384_gs5.md")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(a):
+    """{(kernel, leg): HBM bytes per launch} for the default workload (16384 Plummer patches, ghost size 5),
+    empty for other sizes: the counters were collected on that workload only."""
+    out = {}
+    if a.npatch != 16384 or a.gs != 5 or not os.path.exists(TRAFFIC_FILE):
+        return out
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            w = line.split()
+            if len(w) == 4 and w[0] == "traffic":
+                out[(w[1], w[2])] = float(w[3])
+    return out
 
 
 def parse():
@@ -43,7 +60,9 @@ def parse():
     ap.add_argument("--npatch", type=int, default=16384)
     ap.add_argument("--gs", type=int, default=5)
     ap.add_argument("--check-every", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=512, help="patches timed on the CPU oracle (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=16384,
+                    help="patches timed on the CPU oracle (0: skip; default: the whole 16384-patch workload, "
+                         "about a second on 16 host cores)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--pad", type=int, default=None, help="SOR split-row padding (default: library choice)")
     ap.add_argument("--ppw", type=int, default=None, help="patches per wavefront (default: library choice)")
@@ -262,13 +281,16 @@ def main():
                 "converged": int(((st & pg.STATUS_CONVERGED) != 0).sum()),
                 "fallbacks": int(((st & pg.STATUS_FALLBACK) != 0).sum()), "iters": it}
 
-    def lds_roof(r):
+    traffic = measured_traffic(a)
+
+    def lds_roof(r, leg):
         alg = float(r["iters"].astype(np.float64).sum()) * R ** 3 * LDS_BYTES_PER_UPDATE
         ach = alg / (r["solve_ms"] * 1e-3)
-        hbm = a.npatch * 4.0 * (R ** 3 + g["C"] ** 3 + 10 ** 3 + 4 * 512 + 5 * 512)
+        hbm = a.npatch * 4.0 * (R ** 3 * (2 if leg == "warm" else 1) + g["C"] ** 3 + 10 ** 3 + 4 * 512 + 5 * 512)
         return {"kernel": "sor_uniform_kernel", "bound": "lds", "achieved": ach / 1e9, "peak": LDS_PEAK / 1e9,
-                "unit": "GB/s", "frac": ach / LDS_PEAK, "traffic": None,
+                "unit": "GB/s", "frac": ach / LDS_PEAK, "traffic": traffic.get(("sor_uniform_kernel", leg)),
                 "algorithmic_bytes_per_launch": alg, "launch_ms": r["solve_ms"],
+                "hbm_algorithmic_bytes_per_launch": hbm,
                 "hbm_achieved": hbm / (r["solve_ms"] * 1e-3) / 1e9, "hbm_peak": HBM_PEAK / 1e9}
 
     plain = warm = warm_serial = None
@@ -295,16 +317,19 @@ def main():
             log(f"[bench] no trained weights at {a.weights}: warm-start leg skipped")
     head_r = warm if warm is not None else plain
     # per-kernel rooflines come from the serial legs (each kernel timed alone on the stream)
-    roof_sor = lds_roof(warm_serial if warm_serial is not None else plain)
+    roof_sor = lds_roof(warm_serial, "warm") if warm_serial is not None else lds_roof(plain, "plain")
     roofs = {"sor": roof_sor}
+    if warm_serial is not None and plain is not None:
+        roofs["sor_plain"] = lds_roof(plain, "plain")
     roof = roof_sor
     if warm_serial is not None:
         flop = CNN_MFMA_FLOP * a.npatch
         ach = flop / (warm_serial["network_ms"] * 1e-3)
         roofs["cnn"] = {"kernel": "fused_guess_kernel (interpolation, packing, 6 MFMA convolutions in LDS, guess)",
                         "bound": "mfma", "achieved": ach / 1e12, "peak": MFMA_BF16_PEAK / 1e12, "unit": "TFLOP/s",
-                        "frac": ach / MFMA_BF16_PEAK, "traffic": None, "algorithmic_flop_per_launch": flop,
-                        "launch_ms": warm_serial["network_ms"]}
+                        "frac": ach / MFMA_BF16_PEAK, "traffic": traffic.get(("fused_guess_kernel", "warm")),
+                        "algorithmic_flop_per_launch": flop, "launch_ms": warm_serial["network_ms"],
+                        "hbm_algorithmic_bytes_per_launch": a.npatch * 4.0 * (2 * R ** 3 + g["C"] ** 3)}
         if warm_serial["network_ms"] > warm_serial["solve_ms"]:
             roof = roofs["cnn"]
 
@@ -314,13 +339,15 @@ def main():
 
         n = min(a.cpu_sample, a.npatch)
         nt = len(os.sched_getaffinity(0))
+        nt = min(nt, int(os.environ.get("OMP_NUM_THREADS") or nt))   # the job's CPU share
         p = O.make_params(gs=a.gs, init_mode=O.INIT_ZERO, term_mode=O.TERM_RELRES, check_every=a.check_every,
                           rel_tol=1e-5, sor_min_iter=10, sor_max_iter=1000, sor_omega=1.8, poi_coeff=poi, dt=1e-3)
         t = time.perf_counter()
         _, _, it_cpu, _ = O.solve(rho[:n], cpot[:n], flu[:n], p, nthreads=nt)
         dt = time.perf_counter() - t
         cpu = {"value": n / dt, "unit": "patch-solves/s", "cores": nt, "kind": "port",
-               "sample": f"first {n} of the {a.npatch} patches, zero guess, FP32 oracle, OpenMP over patches",
+               "sample": (f"all {n} patches of the workload" if n == a.npatch else f"first {n} of the {a.npatch} patches")
+                          + ", zero guess, FP32 oracle (C, -O3), OpenMP over patches",
                "mean_iters": float(it_cpu.mean())}
 
     def strip(r):
