@@ -16,6 +16,9 @@ constexpr int kCnnOut = 8;                                   // output channels 
 constexpr int kCnnLayers = 6;
 constexpr int kCnnK = 27 * kCnnChannels;                     // 432
 constexpr int kCnnLayerParams = kCnnChannels * kCnnK + kCnnChannels;   // W [16][432] + b [16]
+// Forward-pass BF16 weight layout: per output channel 15 K steps of 32 = 3 dz x 5 pairs of (dy, dx)
+// positions (the 10th position is a zero pad) x 16 input channels
+constexpr int kCnnKF = 3 * 5 * 2 * kCnnChannels;              // 480
 constexpr int kCnnParams = kCnnLayers * kCnnLayerParams;     // 41568
 constexpr int kCnnVox = kCnnEdge * kCnnEdge * kCnnEdge;      // 512
 constexpr int kCnnPadVox = kCnnPadded * kCnnPadded * kCnnPadded;   // 1000
@@ -35,6 +38,7 @@ hipError_t cnn_conv_bwd_weight(const uint16_t* X, const uint16_t* dH, float* wpa
 hipError_t cnn_reduce_parts(const float* part, int parts, int len, float* dst, hipStream_t s);
 hipError_t cnn_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                     float c1, float c2, hipStream_t s);
+// Wb: [6][16][kCnnKF] forward layout; WT: [6][16][432] per-tap transposed (input-gradient pass)
 hipError_t cnn_weight_copies(const float* P, uint16_t* Wb, uint16_t* WT, hipStream_t s);
 hipError_t cnn_prep(const float* rho, const float* cpot, int n, int gs, float cst, int int_scheme, uint16_t* x,
                     float* scale, hipStream_t s);

@@ -18,10 +18,14 @@
 // convolution: 10^3 x 16 channels in, 8^3 x 16 channels out, K = 27 taps x 16 channels = 432.
 //
 // Convolutions are implicit GEMMs on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16): rows = output
-// voxels (16 per tile), columns = 16 output channels, K ordered tap-major (k = tap*16 + ci) so that each
-// lane's 8 consecutive K values are 8 channels of one input voxel (one 16-byte LDS read).  MFMA lane
-// layout (16x16x32): A[i][k] with i = lane%16, k = 8*(lane/16)+0..7; B[k][j] with j = lane%16,
+// voxels (16 per tile), columns = 16 output channels, K made of (tap, channel) pairs so that each lane's
+// 8 consecutive K values are 8 channels of one input voxel (one 16-byte LDS read).  MFMA lane layout
+// (16x16x32): A[i][k] with i = lane%16, k = 8*(lane/16)+0..7; B[k][j] with j = lane%16,
 // k = 8*(lane/16)+0..7; D[i][j] with j = lane%16, i = 4*(lane/16)+0..3.
+// Forward K order: 15 steps c = dz*5 + s; step c covers the in-plane positions 2s and 2s+1 (position
+// dy*3 + dx, 16 channels each) of tap plane dz; position 9 is a pad with zero weights whose A operand
+// repeats position 8.  Grouping the steps by dz lets the fused kernel read an input plane once and feed it
+// to the three output planes that use it.  The backward passes keep the tap-major order k = tap*16 + ci.
 
 #include <hip/hip_runtime.h>
 
@@ -57,7 +61,10 @@ constexpr int S = kCnnPadded;   // 10: padded input edge
 constexpr int So = kCnnEdge;    // 8: output edge
 constexpr int CH = 16;          // channels
 constexpr int K = 27 * CH;      // 432
-constexpr int KC = (K + 31) / 32;  // 14 MFMA K-steps (the last one half used)
+constexpr int KC = (K + 31) / 32;  // 14 MFMA K-steps of the tap-major order (the last one half used)
+constexpr int KZ = 15;          // forward K steps: 3 dz x 5 position pairs
+constexpr int KF = KZ * 32;     // 480 = kCnnKF
+static_assert(KF == kCnnKF, "forward weight layout");
 constexpr int kTZ = 2;          // output z-planes per workgroup
 constexpr int kSlabs = So / kTZ;
 constexpr int kPlane = S * S * CH;  // bf16 values per padded plane
@@ -77,6 +84,20 @@ __device__ __forceinline__ void load_weights(const u16* __restrict__ W, bf16x8 (
   }
 }
 
+// B operand of the forward layout (row j = lane%16 of [16][480]) for the 15 steps.
+__device__ __forceinline__ void load_weights_fwd(const u16* __restrict__ W, bf16x8 (&wreg)[KZ]) {
+  const int j = threadIdx.x & 15, kq = 8 * ((threadIdx.x & 63) >> 4);
+#pragma unroll
+  for (int c = 0; c < KZ; c++) {
+    Pack8 p;
+    p.u = *reinterpret_cast<const uint4*>(W + j * KF + c * 32 + kq);
+    wreg[c] = p.v;
+  }
+}
+
+// In-plane position (dy*3 + dx) and channel half read by lane group g = lane/16 at position pair s.
+__device__ __forceinline__ int fwd_pos(int s, int g) { return min(2 * s + (g >> 1), 8); }
+
 // ---- forward convolution ------------------------------------------------------------------------------
 // MODE 0: y = relu(h) (layer 1);  MODE 1: y = x_centre + relu(h) (layers 2-5), both written into the
 // interior of a padded [n][10^3][16] BF16 buffer whose border stays zero, with h kept for the backward
@@ -91,10 +112,10 @@ __global__ void __launch_bounds__(kThreads) conv_fwd_kernel(const u16* __restric
   const int n = blockIdx.x / kSlabs, z0 = (blockIdx.x % kSlabs) * kTZ;
   const uint4* src = reinterpret_cast<const uint4*>(X + ((size_t)n * S + z0) * kPlane);
   for (int i = threadIdx.x; i < (kTZ + 2) * kPlane / 8; i += kThreads) smem[i] = src[i];
-  bf16x8 wreg[KC];
-  load_weights(W, wreg);
+  bf16x8 wreg[KZ];
+  load_weights_fwd(W, wreg);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int co = lane & 15, kq = 8 * (lane >> 4);
+  const int co = lane & 15, g = lane >> 4;
   const float b = bias[co];
   __syncthreads();
   constexpr int M = kTZ * So * So;   // 128 output voxels: 8 tiles, 2 per wave
@@ -103,16 +124,10 @@ __global__ void __launch_bounds__(kThreads) conv_fwd_kernel(const u16* __restric
     const int lz = v / (So * So), oy = (v / So) % So, ox = v % So;
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int c = 0; c < KC; c++) {
-      const int kb = c * 32 + kq;
-      const int tap = kb / CH, ci0 = kb % CH;
+    for (int c = 0; c < KZ; c++) {
+      const int dz = c / 5, pos = fwd_pos(c % 5, g), dy = pos / 3, dx = pos % 3;
       Pack8 a;
-      if (tap < 27) {
-        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-        a.u = *reinterpret_cast<const uint4*>(xs + (((lz + dz) * S + oy + dy) * S + ox + dx) * CH + ci0);
-      } else {
-        a.u = make_uint4(0, 0, 0, 0);
-      }
+      a.u = *reinterpret_cast<const uint4*>(xs + (((lz + dz) * S + oy + dy) * S + ox + dx) * CH + 8 * (g & 1));
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, wreg[c], acc, 0, 0, 0);
     }
 #pragma unroll
@@ -317,16 +332,24 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, 
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// BF16 copies of the six weight matrices: W[l] [16][432] as is, and per-tap transposed
+// BF16 copies of the six weight matrices: the forward layout Wb[l][co][c*32 + kk] (step c = dz*5 + s,
+// position 2s + kk/16, channel kk%16; zero at the pad position 9), and per-tap transposed
 // WT[l][ci][tap*16+co] = W[l][co][tap*16+ci] for the input-gradient pass.
 __global__ void weight_copies_kernel(const float* __restrict__ P, u16* __restrict__ Wb, u16* __restrict__ WT) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= kCnnLayers * 16 * K) return;
-  const int l = i / (16 * K), r = i % (16 * K);
-  const float* W = P + (size_t)l * kCnnLayerParams;
-  Wb[i] = f2bf(W[r]);
-  const int ci = r / K, k = r % K, tap = k / CH, co = k % CH;
-  WT[i] = f2bf(W[co * K + tap * CH + ci]);
+  if (i < kCnnLayers * 16 * KF) {
+    const int l = i / (16 * KF), r = i % (16 * KF);
+    const int co = r / KF, k = r % KF, c = k / 32, kk = k % 32;
+    const int dz = c / 5, pos = 2 * (c % 5) + (kk >> 4);
+    const float* W = P + (size_t)l * kCnnLayerParams;
+    Wb[i] = pos < 9 ? f2bf(W[co * K + (dz * 9 + pos) * CH + (kk & 15)]) : (u16)0;
+  }
+  if (i < kCnnLayers * 16 * K) {
+    const int l = i / (16 * K), r = i % (16 * K);
+    const float* W = P + (size_t)l * kCnnLayerParams;
+    const int ci = r / K, k = r % K, tap = k / CH, co = k % CH;
+    WT[i] = f2bf(W[co * K + tap * CH + ci]);
+  }
 }
 
 // ---- input preparation and guess assembly -------------------------------------------------------------
@@ -480,63 +503,72 @@ struct FusedLds {
 };
 
 __device__ __forceinline__ void load_layer(const u16* __restrict__ Wb, const float* __restrict__ P, int l,
-                                           bf16x8 (&w)[KC], float& b) {
-  load_weights(Wb + (size_t)l * 16 * K, w);
+                                           bf16x8 (&w)[KZ], float& b) {
+  load_weights_fwd(Wb + (size_t)l * 16 * KF, w);
   b = P[(size_t)l * kCnnLayerParams + 16 * K + (threadIdx.x & 15)];
 }
 
-// This lane's A-operand offset (bf16 elements, relative to the tile voxel's padded index) for each K step:
-// the K step covers taps 2c and 2c+1; the lane group (lane/16) picks the tap parity and channel half.
-// -1 marks the unused second half of the last step.
-__device__ __forceinline__ void tap_offsets(int (&off)[KC]) {
-  const int kq = 8 * ((threadIdx.x & 63) >> 4);
+// This lane's A-operand offsets (bf16 elements, relative to its row voxel in an input plane) for the five
+// position pairs: position fwd_pos(s, g) of the plane, channel half g & 1 (g = lane/16).
+__device__ __forceinline__ void pair_offsets(int (&off)[5]) {
+  const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
-  for (int c = 0; c < KC; c++) {
-    const int kb = c * 32 + kq;
-    const int tap = kb / CH, half = (kb % CH) >> 3;
-    off[c] = tap < 27 ? half * kHalfP + (((tap / 9) * S + (tap / 3) % 3) * S + tap % 3) * 8 : -1;
+  for (int s = 0; s < 5; s++) {
+    const int pos = fwd_pos(s, g);
+    off[s] = (g & 1) * kHalfP + ((pos / 3) * S + pos % 3) * 8;
   }
 }
 
-// One convolution layer from LDS to LDS (MODE 0/1) or to the FP32 output D in LDS (MODE 2).
+// Epilogue of output plane o for wave `wave`: rows 4g..4g+3 of tile 4o + wave (4 consecutive voxels along x).
 template <int MODE>
-__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KC], float b, float* D,
-                                         const int (&off)[KC]) {
+__device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, const floatx4& acc, float b, int o) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int co = lane & 15, g = lane >> 4;
-  for (int t = wave; t < kCnnVox / 16; t += kThreads / 64) {
-    const int v = t * 16 + (lane & 15);
-    const u16* src = in + (((v >> 6) * S + ((v >> 3) & 7)) * S + (v & 7)) * 8;   // tap (0,0,0), half 0
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int vo0 = (4 * o + wave) * 16 + 4 * g;
+  if (MODE == 2) {
+    if (co < kCnnOut) {
 #pragma unroll
-    for (int c = 0; c < KC; c++) {
-      Pack8 a;
-      if (c < KC - 1) {
-        a.u = *reinterpret_cast<const uint4*>(src + off[c]);
-      } else {
-        const bool live = off[c] >= 0;
-        a.u = *reinterpret_cast<const uint4*>(src + (live ? off[c] : 0));
-        if (!live) a.u = make_uint4(0, 0, 0, 0);
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, wreg[c], acc, 0, 0, 0);
+      for (int r = 0; r < 4; r++) D[(vo0 + r) * kCnnOut + co] = acc[r] + b;
     }
-    // rows 4g..4g+3 of the tile: 4 consecutive voxels along x
-    const int vo0 = t * 16 + 4 * g;
-    if (MODE == 2) {
-      if (co < kCnnOut) {
+  } else {
+    const int q0 = (((vo0 >> 6) + 1) * S + ((vo0 >> 3) & 7) + 1) * S + (vo0 & 7) + 1;
+    const int c0 = (co >> 3) * kHalfP + q0 * 8 + (co & 7);
 #pragma unroll
-        for (int r = 0; r < 4; r++) D[(vo0 + r) * kCnnOut + co] = acc[r] + b;
-      }
-    } else {
-      const int q0 = (((vo0 >> 6) + 1) * S + ((vo0 >> 3) & 7) + 1) * S + (vo0 & 7) + 1;
-      const int c0 = (co >> 3) * kHalfP + q0 * 8 + (co & 7);
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        float o = fmaxf(acc[r] + b, 0.0f);
-        if (MODE == 1) o += bf2f(in[c0 + 8 * r]);
-        out[c0 + 8 * r] = f2bf(o);
-      }
+    for (int r = 0; r < 4; r++) {
+      float v = fmaxf(acc[r] + b, 0.0f);
+      if (MODE == 1) v += bf2f(in[c0 + 8 * r]);
+      out[c0 + 8 * r] = f2bf(v);
     }
+  }
+}
+
+// One convolution layer from LDS to LDS (MODE 0/1) or to the FP32 output D in LDS (MODE 2).  Wave w owns
+// output rows y = 2w, 2w+1 of every plane (tiles 4o + w) and sweeps the ten input planes once: input plane p
+// feeds output planes p (dz = 0), p-1 (dz = 1) and p-2 (dz = 2), so every A operand read from LDS serves up
+// to three MFMAs (50 reads for 120 MFMAs per wave instead of one read per MFMA).  Output plane o still
+// accumulates its 15 steps in the order of conv_fwd_kernel: dz = 0, 1, 2, and s = 0..4 within each.
+template <int MODE>
+__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KZ], float b, float* D,
+                                         const int (&off)[5]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15;
+  const u16* col = in + ((2 * wave + (i >> 3)) * S + (i & 7)) * 8;   // plane 0, row voxel of lane i
+  const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+  floatx4 a0 = zero, a1 = zero;   // output planes p-2 and p-1
+#pragma unroll
+  for (int p = 0; p < S; p++) {
+    Pack8 a[5];
+#pragma unroll
+    for (int s = 0; s < 5; s++) a[s].u = *reinterpret_cast<const uint4*>(col + p * (S * S * 8) + off[s]);
+    floatx4 a2 = zero;            // output plane p
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+      if (p < So) a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[s], a2, 0, 0, 0);
+      if (p >= 1 && p <= So) a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[5 + s], a1, 0, 0, 0);
+      if (p >= 2) a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[10 + s], a0, 0, 0, 0);
+    }
+    if (p >= 2) conv_store<MODE>(in, out, D, a0, b, p - 2);
+    a0 = a1;
+    a1 = a2;
   }
 }
 
@@ -587,7 +619,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   u16* A = reinterpret_cast<u16*>(L.A);
   u16* B = reinterpret_cast<u16*>(L.B);
   // first layer's weights, the coarse cube and this lane's share of the density are requested together
-  bf16x8 wa[KC], wb[KC];
+  bf16x8 wa[KZ], wb[KZ];
   float ba, bb;
   load_layer(Wb, P, 0, wa, ba);
   for (int t = threadIdx.x; t < C3 / 4; t += kThreads)
@@ -645,8 +677,8 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   }
   __syncthreads();   // every lane is done reading the cube in B
   for (int i = threadIdx.x; i < kBufVec; i += kThreads) L.B[i] = make_uint4(0, 0, 0, 0);
-  int off[KC];
-  tap_offsets(off);
+  int off[5];
+  pair_offsets(off);
   __syncthreads();
   load_layer(Wb, P, 1, wb, bb);
   conv_lds<0>(A, B, wa, ba, nullptr, off);     // layer 1: A -> B
@@ -774,7 +806,7 @@ hipError_t cnn_adam(float* p, const float* g, float* m, float* v, int n, float l
 }
 
 hipError_t cnn_weight_copies(const float* P, uint16_t* Wb, uint16_t* WT, hipStream_t s) {
-  const int n = kCnnLayers * 16 * K;
+  const int n = kCnnLayers * 16 * KF;   // covers both copies (KF > K)
   hipLaunchKernelGGL(weight_copies_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, Wb, WT);
   return hipGetLastError();
 }

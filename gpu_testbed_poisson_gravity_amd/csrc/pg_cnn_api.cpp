@@ -25,6 +25,7 @@ using namespace pgamd;
 namespace {
 
 constexpr int kW = kCnnChannels * kCnnK;   // weights per layer
+constexpr int kWF = kCnnChannels * kCnnKF; // forward-layout BF16 weights per layer
 constexpr int kMseBlocks = 64;
 constexpr int kMaxReduceBlocks = 4096;
 constexpr int kMaxChunks = 16;               // pgamd_cnn_solve_device
@@ -38,7 +39,7 @@ struct pgamd_cnn {
   int device = 0;
   int step = 0;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  uint16_t* wb = nullptr;   // BF16 W1..W6 as stored
+  uint16_t* wb = nullptr;   // BF16 W1..W6 in the forward layout [16][480]
   uint16_t* wt = nullptr;   // BF16 W1..W6 transposed per tap (input-gradient pass)
   // workspace
   int cap = 0;              // samples
@@ -121,11 +122,11 @@ int refresh_copies(pgamd_cnn* n, hipStream_t s) {
 int forward(pgamd_cnn* n, const uint16_t* x0, int count, bool keep_h, hipStream_t s) {
   const uint16_t* in = x0;
   for (int l = 1; l <= 5; l++) {
-    CK(cnn_conv_fwd(in, n->wb + (size_t)(l - 1) * kW, n->p + ob(l), n->y[l], keep_h ? n->h[l] : nullptr, nullptr,
+    CK(cnn_conv_fwd(in, n->wb + (size_t)(l - 1) * kWF, n->p + ob(l), n->y[l], keep_h ? n->h[l] : nullptr, nullptr,
                     count, l == 1 ? 0 : 1, s));
     in = n->y[l];
   }
-  CK(cnn_conv_fwd(in, n->wb + (size_t)5 * kW, n->p + ob(6), nullptr, nullptr, n->d, count, 2, s));
+  CK(cnn_conv_fwd(in, n->wb + (size_t)5 * kWF, n->p + ob(6), nullptr, nullptr, n->d, count, 2, s));
   return PGAMD_OK;
 }
 
@@ -178,7 +179,7 @@ int pgamd_cnn_create(pgamd_context* ctx, uint64_t seed, pgamd_cnn** out) {
   n->device = ctx_device(ctx);
   if (hipSetDevice(n->device) != hipSuccess || hipMalloc(&n->p, kCnnParams * 4) != hipSuccess ||
       hipMalloc(&n->g, kCnnParams * 4) != hipSuccess || hipMalloc(&n->m, kCnnParams * 4) != hipSuccess ||
-      hipMalloc(&n->v, kCnnParams * 4) != hipSuccess || hipMalloc(&n->wb, kCnnLayers * kW * 2) != hipSuccess ||
+      hipMalloc(&n->v, kCnnParams * 4) != hipSuccess || hipMalloc(&n->wb, kCnnLayers * kWF * 2) != hipSuccess ||
       hipMalloc(&n->wt, kCnnLayers * kW * 2) != hipSuccess) {
     pgamd_cnn_destroy(n);
     return set_error(PGAMD_ERR_NOMEM, "CNN parameter allocation failed");
