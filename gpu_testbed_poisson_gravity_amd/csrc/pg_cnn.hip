@@ -484,23 +484,31 @@ __global__ void __launch_bounds__(kThreads) guess_kernel(const float* __restrict
 //
 // * An activation buffer is split into two half-planes of 8 channels, [half][voxel][8] (16 KB each), so
 //   that the 16-byte A-operand reads of 8 consecutive lanes (8 consecutive voxels along x) are one
-//   contiguous 128-byte span: no bank conflicts.
-// * Global traffic is issued early and coalesced: the coarse cube is staged once (float4) and kept in LDS
-//   for the final interpolation, the density is read with float4 loads and scattered into the packed
-//   input, and each layer's weights are loaded into registers while the previous layer computes.
+//   contiguous 128-byte span.  The two rows of a tile start 160 bytes apart, so their spans share 32
+//   bytes of banks: measured about 3.6 extra LDS cycles per read; a pitch-8 layout without the x border
+//   removes that overlap but needs a select per read and measured slower.
+// * Global traffic is issued early and coalesced: the coarse cube is staged once (float4), the density is
+//   read with float4 loads and scattered into the packed input, and the interpolated interior is kept in
+//   LDS (over the coarse staging, which is dead by then) for the final guess.
 
 // Half-plane stride (bf16 values).  Unpadded: measured on MI355X, padding the stride by 32, 64, 128 or
 // 256 bytes made the kernel 2-13% slower (more LDS bank-conflict cycles on the 16-byte A-operand reads).
 constexpr int kHalfP = kCnnPadVox * 8;
 constexpr int kBufVec = kHalfP * 2 / 8;      // uint4 per activation buffer
 constexpr int kCoarseMax = 12 * 12 * 12;     // C^3 at ghost size 5
+constexpr int kInteriorMax = 16 * 16 * 16;   // R^3 at ghost size 5
 
+// 80.4 KB: two workgroups fit in the 160 KB of LDS of a CU
 struct FusedLds {
   uint4 A[kBufVec];
   uint4 B[kBufVec];
-  float cs[kCoarseMax];
+  union {
+    float cs[kCoarseMax];         // coarse cube, until the interpolation is done
+    float in[kInteriorMax];       // then the interpolated interior (solve cells) for the guess
+  } c;
   float red[kThreads / 64];
 };
+static_assert(sizeof(FusedLds) <= 160 * 1024 / 2, "two workgroups per CU");
 
 __device__ __forceinline__ void load_layer(const u16* __restrict__ Wb, const float* __restrict__ P, int l,
                                            bf16x8 (&w)[KZ], float& b) {
@@ -613,7 +621,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   constexpr int N = g.N, R = g.R, C = g.C, OFF = 5 - GS;
   constexpr int R3 = R * R * R, C3 = C * C * C;
   static_assert(N * N * N * 4 <= kBufVec * 16 && kCnnOutputs * 4 <= kBufVec * 16, "scratch must fit a buffer");
-  static_assert(C3 <= kCoarseMax && R3 % 4 == 0 && C3 % 4 == 0, "float4 staging");
+  static_assert(C3 <= kCoarseMax && R3 <= kInteriorMax && R3 % 4 == 0 && C3 % 4 == 0, "float4 staging");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float* cube = reinterpret_cast<float*>(L.B);
   u16* A = reinterpret_cast<u16*>(L.A);
@@ -623,7 +631,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   float ba, bb;
   load_layer(Wb, P, 0, wa, ba);
   for (int t = threadIdx.x; t < C3 / 4; t += kThreads)
-    reinterpret_cast<float4*>(L.cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
+    reinterpret_cast<float4*>(L.c.cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
   constexpr int RV = (R3 / 4 + kThreads - 1) / kThreads;   // float4 of density per thread
   float4 rv[RV];
 #pragma unroll
@@ -632,7 +640,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
     rv[k] = i4 < R3 / 4 ? reinterpret_cast<const float4*>(rho)[i4] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
-  interp_cube_lds<GS>(L.cs, int_scheme, cube);
+  interp_cube_lds<GS>(L.c.cs, int_scheme, cube);   // ends with a barrier: the coarse staging is dead
   float mx = 0.0f;
   for (int t = threadIdx.x; t < N * N * N; t += kThreads) mx = fmaxf(mx, fabsf(cube[t]));
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
@@ -672,6 +680,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
         const int d = ((ez & 1) * 2 + (ey & 1)) * 2 + (ex & 1);
         const float r = interp_residual<N>(cube, lx + 1, ly + 1, lz + 1, cst * e4[e]);
         A[((d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)) * 2 + 1] = f2bf((r * inv) * kResScale);
+        L.c.in[idx] = cube[((lz + 1) * N + ly + 1) * N + lx + 1];
       }
     }
   }
@@ -699,13 +708,12 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   float* D = reinterpret_cast<float*>(L.A);
   conv_lds<2>(B, nullptr, wb, bb, D, off);     // layer 6: B -> D (in A)
   __syncthreads();
-  interp_cube_lds<GS>(L.cs, int_scheme, cube);  // B is free again
   for (int t = threadIdx.x; t < R3; t += kThreads) {
     const int x = t % R, y = (t / R) % R, z = t / (R * R);
     const int ix = x + OFF, iy = y + OFF, iz = z + OFF;
     const int o = ((iz >> 1) * So + (iy >> 1)) * So + (ix >> 1);
     const int p = ((iz & 1) * 2 + (iy & 1)) * 2 + (ix & 1);
-    guess[t] = cube[((z + 1) * N + y + 1) * N + x + 1] + s * D[o * kCnnOut + p];
+    guess[t] = L.c.in[t] + s * D[o * kCnnOut + p];
   }
 }
 
