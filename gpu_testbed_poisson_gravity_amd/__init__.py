@@ -106,6 +106,7 @@ SIGNATURES = {
     "pgamd_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pgamd_graph_destroy": (C.c_int, [C.c_void_p]),
     "pgamd_cnn_num_params": (C.c_int, []),
+    "pgamd_cnn_set_loss": (C.c_int, [C.c_void_p, C.c_float]),
     "pgamd_cnn_create": (C.c_int, [_CTX, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pgamd_cnn_destroy": (C.c_int, [C.c_void_p]),
     "pgamd_cnn_get_params": (C.c_int, [C.c_void_p, _FP, C.c_int]),
@@ -424,6 +425,10 @@ class CNN:
 
     def forward(self, x0, count, out, stream=None):
         _check(load().pgamd_cnn_forward(self.h, _vp(x0), count, _vp(out), stream))
+
+    def set_loss(self, res_weight: float = 0.0):
+        """Loss = MSE + res_weight * mean squared residual the error leaves in the guess (0: plain MSE)."""
+        _check(load().pgamd_cnn_set_loss(self.h, res_weight))
 
     def loss_grad(self, x0, target, count, grad=None, stream=None) -> float:
         loss = C.c_float()

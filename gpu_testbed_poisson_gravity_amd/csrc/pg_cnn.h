@@ -29,6 +29,9 @@ constexpr int kCnnOutputs = kCnnVox * kCnnOut;               // fp32 values per 
 hipError_t cnn_conv_fwd(const uint16_t* X, const uint16_t* W, const float* b, uint16_t* Y, uint16_t* H, float* D,
                         int n, int mode, hipStream_t s);
 hipError_t cnn_mse(const float* D, const float* T, float* dY, float* partial, int nvox, int nblocks, hipStream_t s);
+// Adds the residual term of the loss (weight w) to dY and to the nblocks loss partials cnn_mse wrote.
+hipError_t cnn_res_loss(const float* D, const float* T, float* dY, float* partial, int n, float w, int nblocks,
+                        hipStream_t s);
 hipError_t cnn_relu_bwd(const float* dY, const uint16_t* H, uint16_t* dH, float* bpart, size_t count, int nblocks,
                         hipStream_t s);
 hipError_t cnn_conv_bwd_data(const uint16_t* dH, const uint16_t* WT, const float* dYres, float* dX, int n,

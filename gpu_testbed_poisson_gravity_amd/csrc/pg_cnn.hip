@@ -177,6 +177,60 @@ __global__ void __launch_bounds__(kThreads) mse_kernel(const float* __restrict__
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// 7-point stencil (six neighbours - 6 a) at cell f of a 16^3 frame held in LDS, zero outside the frame.
+__device__ __forceinline__ float stencil16(const float* a, int f) {
+  constexpr int F = 16;
+  const int x = f % F, y = (f / F) % F, z = f / (F * F);
+  const float zp = z + 1 < F ? a[f + F * F] : 0.0f, zm = z > 0 ? a[f - F * F] : 0.0f;
+  const float yp = y + 1 < F ? a[f + F] : 0.0f, ym = y > 0 ? a[f - F] : 0.0f;
+  const float xp = x + 1 < F ? a[f + 1] : 0.0f, xm = x > 0 ? a[f - 1] : 0.0f;
+  return (((((zp + zm) + yp) + ym) + xp) + xm) - 6.0f * a[f];
+}
+
+// Residual term of the loss, run after mse_kernel on the same stream: per sample, e = D - T on the 16^3
+// frame (fine cell 2o + p of output voxel o, sub-cell p) and r = stencil(e), the residual the error leaves in
+// the guess.  Adds w * 2 / (n * 4096) * stencil(r) to dY (the stencil is symmetric with the zero exterior)
+// and w * sum(r^2) to this block's loss partial.  Block b handles samples b, b + gridDim.x, ...
+__global__ void __launch_bounds__(kThreads) res_loss_kernel(const float* __restrict__ D, const float* __restrict__ T,
+                                                            float* __restrict__ dY, float* __restrict__ partial,
+                                                            int n, float w) {
+  constexpr int F = 16, F3 = F * F * F;
+  static_assert(F3 == kCnnOutputs, "one frame cell per output value");
+  __shared__ float e[F3], r[F3];
+  __shared__ float red[kThreads];
+  const float k = w * (2.0f / ((float)n * kCnnOutputs));
+  float acc = 0.0f;
+  for (int smp = blockIdx.x; smp < n; smp += gridDim.x) {
+    const float* d = D + (size_t)smp * kCnnOutputs;
+    const float* t = T + (size_t)smp * kCnnOutputs;
+    for (int f = threadIdx.x; f < F3; f += kThreads) {
+      const int x = f % F, y = (f / F) % F, z = f / (F * F);
+      const int o = ((z >> 1) * So + (y >> 1)) * So + (x >> 1), p = ((z & 1) * 2 + (y & 1)) * 2 + (x & 1);
+      e[f] = d[o * kCnnOut + p] - t[o * kCnnOut + p];
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < F3; f += kThreads) {
+      const float v = stencil16(e, f);
+      r[f] = v;
+      acc += v * v;
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < F3; f += kThreads) {
+      const int x = f % F, y = (f / F) % F, z = f / (F * F);
+      const int o = ((z >> 1) * So + (y >> 1)) * So + (x >> 1), p = ((z & 1) * 2 + (y & 1)) * 2 + (x & 1);
+      dY[((size_t)smp * kCnnVox + o) * CH + p] += k * stencil16(r, f);
+    }
+    __syncthreads();   // e and r are rewritten for the next sample
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] += w * red[0];
+}
+
 // dH = dY * (H > 0) (or dY when H is null: the linear output layer), stored BF16 for the matrix cores;
 // per-block partial bias gradients (thread t always handles channel t % 16).
 __global__ void __launch_bounds__(kThreads) relu_bwd_kernel(const float* __restrict__ dY, const u16* __restrict__ H,
@@ -497,6 +551,9 @@ constexpr int kHalfP = kCnnPadVox * 8;
 constexpr int kBufVec = kHalfP * 2 / 8;      // uint4 per activation buffer
 constexpr int kCoarseMax = 12 * 12 * 12;     // C^3 at ghost size 5
 constexpr int kInteriorMax = 16 * 16 * 16;   // R^3 at ghost size 5
+constexpr int kFT = 256;                     // threads of a fused workgroup (4 or 8 waves)
+constexpr int kZSplit = kFT / 256;           // groups of output planes the waves split a layer into
+static_assert(kFT == 256 || kFT == 512, "4 row pairs x 1 or 2 plane groups");
 
 // 80.4 KB: two workgroups fit in the 160 KB of LDS of a CU
 struct FusedLds {
@@ -506,7 +563,7 @@ struct FusedLds {
     float cs[kCoarseMax];         // coarse cube, until the interpolation is done
     float in[kInteriorMax];       // then the interpolated interior (solve cells) for the guess
   } c;
-  float red[kThreads / 64];
+  float red[kFT / 64];
 };
 static_assert(sizeof(FusedLds) <= 160 * 1024 / 2, "two workgroups per CU");
 
@@ -527,12 +584,13 @@ __device__ __forceinline__ void pair_offsets(int (&off)[5]) {
   }
 }
 
-// Epilogue of output plane o for wave `wave`: rows 4g..4g+3 of tile 4o + wave (4 consecutive voxels along x).
+// Epilogue of output plane o for row pair rp: rows 4g..4g+3 of tile 4o + rp (4 consecutive voxels along x).
 template <int MODE>
-__device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, const floatx4& acc, float b, int o) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, const floatx4& acc, float b, int o,
+                                           int rp) {
+  const int lane = threadIdx.x & 63;
   const int co = lane & 15, g = lane >> 4;
-  const int vo0 = (4 * o + wave) * 16 + 4 * g;
+  const int vo0 = (4 * o + rp) * 16 + 4 * g;
   if (MODE == 2) {
     if (co < kCnnOut) {
 #pragma unroll
@@ -551,30 +609,33 @@ __device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, co
 }
 
 // One convolution layer from LDS to LDS (MODE 0/1) or to the FP32 output D in LDS (MODE 2).  Wave w owns
-// output rows y = 2w, 2w+1 of every plane (tiles 4o + w) and sweeps the ten input planes once: input plane p
-// feeds output planes p (dz = 0), p-1 (dz = 1) and p-2 (dz = 2), so every A operand read from LDS serves up
-// to three MFMAs (50 reads for 120 MFMAs per wave instead of one read per MFMA).  Output plane o still
+// output rows y = 2rp, 2rp+1 (rp = w % 4) of the NO = 8 / kZSplit output planes P0.. of its plane group and
+// sweeps the NO + 2 input planes they read once: input plane P0 + q feeds output planes P0 + q (dz = 0),
+// P0 + q - 1 (dz = 1) and P0 + q - 2 (dz = 2), so every A operand read from LDS serves up to three MFMAs
+// (with 4 waves: 50 reads for 120 MFMAs per wave instead of one read per MFMA).  Output plane o still
 // accumulates its 15 steps in the order of conv_fwd_kernel: dz = 0, 1, 2, and s = 0..4 within each.
 template <int MODE>
 __device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KZ], float b, float* D,
                                          const int (&off)[5]) {
+  constexpr int NO = So / kZSplit, NQ = NO + 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15;
-  const u16* col = in + ((2 * wave + (i >> 3)) * S + (i & 7)) * 8;   // plane 0, row voxel of lane i
+  const int rp = wave & 3, P0 = (wave >> 2) * NO;
+  const u16* col = in + ((P0 * S + 2 * rp + (i >> 3)) * S + (i & 7)) * 8;   // plane P0, row voxel of lane i
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
-  floatx4 a0 = zero, a1 = zero;   // output planes p-2 and p-1
+  floatx4 a0 = zero, a1 = zero;   // output planes P0+q-2 and P0+q-1
 #pragma unroll
-  for (int p = 0; p < S; p++) {
+  for (int q = 0; q < NQ; q++) {
     Pack8 a[5];
 #pragma unroll
-    for (int s = 0; s < 5; s++) a[s].u = *reinterpret_cast<const uint4*>(col + p * (S * S * 8) + off[s]);
-    floatx4 a2 = zero;            // output plane p
+    for (int s = 0; s < 5; s++) a[s].u = *reinterpret_cast<const uint4*>(col + q * (S * S * 8) + off[s]);
+    floatx4 a2 = zero;            // output plane P0+q
 #pragma unroll
     for (int s = 0; s < 5; s++) {
-      if (p < So) a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[s], a2, 0, 0, 0);
-      if (p >= 1 && p <= So) a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[5 + s], a1, 0, 0, 0);
-      if (p >= 2) a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[10 + s], a0, 0, 0, 0);
+      if (q < NO) a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[s], a2, 0, 0, 0);
+      if (q >= 1 && q <= NO) a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[5 + s], a1, 0, 0, 0);
+      if (q >= 2) a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[10 + s], a0, 0, 0, 0);
     }
-    if (p >= 2) conv_store<MODE>(in, out, D, a0, b, p - 2);
+    if (q >= 2) conv_store<MODE>(in, out, D, a0, b, P0 + q - 2, rp);
     a0 = a1;
     a1 = a2;
   }
@@ -582,7 +643,7 @@ __device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (
 
 // Zero the border voxels (any coordinate 0 or 9) of both half-planes of a padded activation buffer.
 __device__ __forceinline__ void zero_border(uint4* buf) {
-  for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kThreads) {
+  for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kFT) {
     const int h = i / kCnnPadVox, vox = i % kCnnPadVox;
     const int x = vox % S, y = (vox / S) % S, z = vox / (S * S);
     if (x == 0 || x == S - 1 || y == 0 || y == S - 1 || z == 0 || z == S - 1)
@@ -595,7 +656,7 @@ template <int GS>
 __device__ void interp_cube_lds(const float* cs, int int_scheme, float* cube) {
   constexpr Geo g = geo_of(GS);
   constexpr int N = g.N, C = g.C, U = g.U, NC = C - 2;
-  for (int q = threadIdx.x; q < NC * NC * NC; q += kThreads) {
+  for (int q = threadIdx.x; q < NC * NC * NC; q += kFT) {
     const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
     float v[3][3][3];
     for (int dk = 0; dk < 3; dk++)
@@ -630,28 +691,30 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   bf16x8 wa[KZ], wb[KZ];
   float ba, bb;
   load_layer(Wb, P, 0, wa, ba);
-  for (int t = threadIdx.x; t < C3 / 4; t += kThreads)
+  for (int t = threadIdx.x; t < C3 / 4; t += kFT)
     reinterpret_cast<float4*>(L.c.cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
-  constexpr int RV = (R3 / 4 + kThreads - 1) / kThreads;   // float4 of density per thread
+  constexpr int RV = (R3 / 4 + kFT - 1) / kFT;   // float4 of density per thread
   float4 rv[RV];
 #pragma unroll
   for (int k = 0; k < RV; k++) {
-    const int i4 = threadIdx.x + k * kThreads;
+    const int i4 = threadIdx.x + k * kFT;
     rv[k] = i4 < R3 / 4 ? reinterpret_cast<const float4*>(rho)[i4] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
   interp_cube_lds<GS>(L.c.cs, int_scheme, cube);   // ends with a barrier: the coarse staging is dead
   float mx = 0.0f;
-  for (int t = threadIdx.x; t < N * N * N; t += kThreads) mx = fmaxf(mx, fabsf(cube[t]));
+  for (int t = threadIdx.x; t < N * N * N; t += kFT) mx = fmaxf(mx, fabsf(cube[t]));
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
   if (lane == 0) L.red[wave] = mx;
   __syncthreads();
-  mx = fmaxf(fmaxf(L.red[0], L.red[1]), fmaxf(L.red[2], L.red[3]));
+  mx = L.red[0];
+#pragma unroll
+  for (int w = 1; w < kFT / 64; w++) mx = fmaxf(mx, L.red[w]);
   const float s = mx > 0.0f ? mx : 1.0f;
   const float inv = 1.0f / s;
   // packed input, pass 1: interpolated potential (low halves), zero density (high halves)
   uint32_t* A32 = reinterpret_cast<uint32_t*>(A);
-  for (int vox = threadIdx.x; vox < kCnnPadVox; vox += kThreads) {
+  for (int vox = threadIdx.x; vox < kCnnPadVox; vox += kFT) {
     const int cx = vox % S, cy = (vox / S) % S, cz = vox / (S * S);
 #pragma unroll
     for (int d = 0; d < 8; d++) {
@@ -667,7 +730,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   // read above)
 #pragma unroll
   for (int k = 0; k < RV; k++) {
-    const int i4 = threadIdx.x + k * kThreads;
+    const int i4 = threadIdx.x + k * kFT;
     if (i4 < R3 / 4) {
       const float e4[4] = {rv[k].x, rv[k].y, rv[k].z, rv[k].w};
 #pragma unroll
@@ -685,7 +748,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
     }
   }
   __syncthreads();   // every lane is done reading the cube in B
-  for (int i = threadIdx.x; i < kBufVec; i += kThreads) L.B[i] = make_uint4(0, 0, 0, 0);
+  for (int i = threadIdx.x; i < kBufVec; i += kFT) L.B[i] = make_uint4(0, 0, 0, 0);
   int off[5];
   pair_offsets(off);
   __syncthreads();
@@ -708,7 +771,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   float* D = reinterpret_cast<float*>(L.A);
   conv_lds<2>(B, nullptr, wb, bb, D, off);     // layer 6: B -> D (in A)
   __syncthreads();
-  for (int t = threadIdx.x; t < R3; t += kThreads) {
+  for (int t = threadIdx.x; t < R3; t += kFT) {
     const int x = t % R, y = (t / R) % R, z = t / (R * R);
     const int ix = x + OFF, iy = y + OFF, iz = z + OFF;
     const int o = ((iz >> 1) * So + (iy >> 1)) * So + (ix >> 1);
@@ -728,7 +791,7 @@ __device__ __forceinline__ void fused_dispatch(int gs, const float* r, const flo
   }
 }
 
-__global__ void __launch_bounds__(kThreads) fused_guess_kernel(const float* __restrict__ rho,
+__global__ void __launch_bounds__(kFT, kFT / 128) fused_guess_kernel(const float* __restrict__ rho,
                                                                const float* __restrict__ cpot,
                                                                const u16* __restrict__ Wb, const float* __restrict__ P,
                                                                int npatch, int gs, float cst, int int_scheme,
@@ -745,7 +808,7 @@ __global__ void __launch_bounds__(kThreads) fused_guess_kernel(const float* __re
 
 // Mixed ghost sizes: patch p reads its ghost size and packed offsets from the plan's descriptors; the
 // guess is packed like the density (offset rho_off).
-__global__ void __launch_bounds__(kThreads) fused_guess_mixed_kernel(const float* __restrict__ rho,
+__global__ void __launch_bounds__(kFT, kFT / 128) fused_guess_mixed_kernel(const float* __restrict__ rho,
                                                                      const float* __restrict__ cpot,
                                                                      const PatchDesc* __restrict__ desc,
                                                                      const u16* __restrict__ Wb,
@@ -780,6 +843,12 @@ hipError_t cnn_conv_fwd(const uint16_t* X, const uint16_t* W, const float* b, ui
 
 hipError_t cnn_mse(const float* D, const float* T, float* dY, float* partial, int nvox, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(mse_kernel, dim3(nblocks), dim3(kThreads), 0, s, D, T, dY, partial, nvox);
+  return hipGetLastError();
+}
+
+hipError_t cnn_res_loss(const float* D, const float* T, float* dY, float* partial, int n, float w, int nblocks,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(res_loss_kernel, dim3(nblocks), dim3(kThreads), 0, s, D, T, dY, partial, n, w);
   return hipGetLastError();
 }
 
@@ -829,14 +898,14 @@ hipError_t cnn_prep(const float* rho, const float* cpot, int n, int gs, float cs
 
 hipError_t cnn_fused_guess(const float* rho, const float* cpot, const uint16_t* Wb, const float* P, int n, int gs,
                            float cst, int int_scheme, float* guess, hipStream_t s) {
-  hipLaunchKernelGGL(fused_guess_kernel, dim3(n), dim3(kThreads), 0, s, rho, cpot, Wb, P, n, gs, cst, int_scheme,
+  hipLaunchKernelGGL(fused_guess_kernel, dim3(n), dim3(kFT), 0, s, rho, cpot, Wb, P, n, gs, cst, int_scheme,
                      guess);
   return hipGetLastError();
 }
 
 hipError_t cnn_fused_guess_mixed(const float* rho, const float* cpot, const PatchDesc* desc, const uint16_t* Wb,
                                  const float* P, int n, float cst, int int_scheme, float* guess, hipStream_t s) {
-  hipLaunchKernelGGL(fused_guess_mixed_kernel, dim3(n), dim3(kThreads), 0, s, rho, cpot, desc, Wb, P, n, cst,
+  hipLaunchKernelGGL(fused_guess_mixed_kernel, dim3(n), dim3(kFT), 0, s, rho, cpot, desc, Wb, P, n, cst,
                      int_scheme, guess);
   return hipGetLastError();
 }

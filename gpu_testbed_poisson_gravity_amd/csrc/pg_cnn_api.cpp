@@ -38,6 +38,7 @@ int ob(int l) { return ow(l) + kW; }
 struct pgamd_cnn {
   int device = 0;
   int step = 0;
+  float res_weight = 0.0f;  // weight of the residual term of the loss (pgamd_cnn_set_loss)
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t* wb = nullptr;   // BF16 W1..W6 in the forward layout [16][480]
   uint16_t* wt = nullptr;   // BF16 W1..W6 transposed per tap (input-gradient pass)
@@ -130,7 +131,8 @@ int forward(pgamd_cnn* n, const uint16_t* x0, int count, bool keep_h, hipStream_
   return PGAMD_OK;
 }
 
-// Forward + backward; gradients (mean-squared-error loss) go to n->g, loss partials to lossp.
+// Forward + backward; gradients (mean squared error, plus the residual term when res_weight > 0) go to n->g,
+// loss partials to lossp.
 int forward_backward(pgamd_cnn* n, const uint16_t* x0, const float* target, int count, hipStream_t s) {
   int rc = forward(n, x0, count, true, s);
   if (rc != PGAMD_OK) return rc;
@@ -138,6 +140,7 @@ int forward_backward(pgamd_cnn* n, const uint16_t* x0, const float* target, int 
   const int nb = reduce_blocks(act);
   const int wblocks = cnn_bwd_weight_blocks(count);
   CK(cnn_mse(n->d, target, n->dy6, n->lossp, count * kCnnVox, kMseBlocks, s));
+  if (n->res_weight > 0.0f) CK(cnn_res_loss(n->d, target, n->dy6, n->lossp, count, n->res_weight, kMseBlocks, s));
   const float* dy = n->dy6;
   float* bufs[2] = {n->dyA, n->dyB};
   int which = 0;
@@ -171,6 +174,14 @@ float read_loss(pgamd_cnn* n, int count, hipStream_t s) {
 extern "C" {
 
 int pgamd_cnn_num_params(void) { return kCnnParams; }
+
+int pgamd_cnn_set_loss(pgamd_cnn* n, float res_weight) {
+  if (!n) return set_error(PGAMD_ERR_ARG, "null network");
+  if (!(res_weight >= 0.0f) || !std::isfinite(res_weight))
+    return set_error(PGAMD_ERR_ARG, "res_weight must be finite and >= 0");
+  n->res_weight = res_weight;
+  return PGAMD_OK;
+}
 
 int pgamd_cnn_create(pgamd_context* ctx, uint64_t seed, pgamd_cnn** out) {
   if (!ctx || !out) return set_error(PGAMD_ERR_ARG, "null argument");

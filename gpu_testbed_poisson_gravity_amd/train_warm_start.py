@@ -1,7 +1,7 @@
 """Train the warm-start CNN on patches solved by this library's own SOR solver, then evaluate it.
 
     python -m gpu_testbed_poisson_gravity_amd.train_warm_start --patches 200000 --steps 20000 --batch 256 \
-        --lr 1e-3 --out gpu_testbed_poisson_gravity_amd/weights/warm_start_gs5.npy
+        --lr 1e-3 --res-weight 1.0 --out gpu_testbed_poisson_gravity_amd/weights/warm_start_gs5.npy
 
 Data: ghost size 5 (18^3 solve cube), density fields half sums of 1..8 Plummer spheres and half lognormal
 (sigma 1.5, correlation length 3 cells), generated on the host cores before the GPU is touched.  Targets:
@@ -147,6 +147,8 @@ def main(argv=None):
     ap.add_argument("--eval-patches", type=int, default=16384)
     ap.add_argument("--time-limit", type=float, default=1e9, help="stop training after this many seconds")
     ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--res-weight", type=float, default=0.0,
+                    help="weight of the residual term of the loss (pgamd_cnn_set_loss; 0: mean squared error only)")
     ap.add_argument("--init", default=None, help="start from these weights (.npy)")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights", "warm_start_gs5.npy"))
     a = ap.parse_args(argv)
@@ -166,6 +168,7 @@ def main(argv=None):
     load()
     ctx = Context(0)
     net = CNN(ctx, seed=a.seed)
+    net.set_loss(a.res_weight)
     ctx.set_options(pot_ghost_size=GS)
     d_rho, d_cpot = torch.as_tensor(rho).cuda(), torch.as_tensor(cpot).cuda()
     del rho, cpot
@@ -224,6 +227,7 @@ def main(argv=None):
               f"cnn {k['mean_iters']:.1f} (fallbacks {k['fallbacks']}); time zero {z['ms']:.2f} ms cnn "
               f"{k['ms']:.2f} ms", flush=True)
     record = {"patches": n, "lognormal_frac": a.lognormal_frac, "steps": done, "batch": a.batch, "lr": a.lr,
+              "res_weight": a.res_weight,
               "lr_schedule": "constant, then linear decay to lr/10 over the last quarter", "target_rms": tscale,
               "seed": a.seed, "train_seconds": train_s, "ms_per_step": 1e3 * train_s / max(done, 1),
               "loss_curve": curve, "target_solver": {"rel_tol": 1e-6, "mean_iters": float(it_t.mean())},

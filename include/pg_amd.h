@@ -200,8 +200,13 @@ int pgamd_cnn_set_params(pgamd_cnn* net, const float* host_params, int count);
 int pgamd_cnn_prepare(pgamd_context* ctx, const float* d_rho, const float* d_pot_in, int npatch, int ghost_size,
                       const pgamd_solver_args* args, uint16_t* d_x0, float* d_scale, void* stream);
 int pgamd_cnn_forward(pgamd_cnn* net, const uint16_t* d_x0, int count, float* d_out, void* stream);
-/* Mean-squared-error loss against d_target ([n][8^3][8]) and its gradient (FP32, 41568 values, may be NULL);
- * *loss (may be NULL) is read back synchronously. */
+/* Training loss: mean squared error of the output against d_target ([n][8^3][8]) plus res_weight times the mean
+ * squared residual that error leaves in the guess, r = (sum of the six neighbours - 6 e) on the 16^3 frame
+ * (e = output - target at frame cell 2o + p, zero outside), both means over the n*4096 outputs.  res_weight = 0
+ * (the default) is the plain mean squared error.  Error if res_weight is negative or not finite. */
+int pgamd_cnn_set_loss(pgamd_cnn* net, float res_weight);
+/* Loss (see pgamd_cnn_set_loss) against d_target ([n][8^3][8]) and its gradient (FP32, 41568 values, may be
+ * NULL); *loss (may be NULL) is read back synchronously. */
 int pgamd_cnn_loss_grad(pgamd_cnn* net, const uint16_t* d_x0, const float* d_target, int count, float* loss,
                         float* d_grad, void* stream);
 /* One Adam step (beta 0.9/0.999, eps 1e-8) on a batch. */

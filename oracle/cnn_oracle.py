@@ -1,4 +1,6 @@
-"""CPU oracle of the warm-start CNN (input packing, forward, mean-squared-error loss and backward), numpy.
+"""CPU oracle of the warm-start CNN (input packing, forward, loss and backward), numpy.  The loss is the mean
+squared error of the correction plus an optional weight times the mean squared residual that the error leaves
+in the guess (`loss_grad`, `res_weight`).
 
 TEST INFRASTRUCTURE ONLY: used by tests/ to check the HIP kernels.  The network has no reference
 counterpart; its definition is the one stated in include/pg_amd.h (pgamd_cnn_*):
@@ -154,8 +156,18 @@ def forward(p, x0, bf16=True):
     return D.reshape(-1, 512, NOUT), cache
 
 
-def loss_grad(p, x0, target, bf16=True):
-    """Mean-squared-error loss over all n*512*8 outputs and its gradient (parameter vector order)."""
+def stencil16(e):
+    """7-point operator (sum of the six neighbours - 6 e) on [n,16,16,16] with zeros outside: the SOR residual
+    operator applied to a correction that vanishes on the Dirichlet layer."""
+    f = np.pad(np.asarray(e, np.float64), ((0, 0), (1, 1), (1, 1), (1, 1)))
+    return (f[:, 2:, 1:-1, 1:-1] + f[:, :-2, 1:-1, 1:-1] + f[:, 1:-1, 2:, 1:-1] + f[:, 1:-1, :-2, 1:-1]
+            + f[:, 1:-1, 1:-1, 2:] + f[:, 1:-1, 1:-1, :-2]) - 6.0 * f[:, 1:-1, 1:-1, 1:-1]
+
+
+def loss_grad(p, x0, target, bf16=True, res_weight=0.0):
+    """Loss over all n*512*8 outputs and its gradient (parameter vector order):
+    L = mean(e^2) + res_weight * sum(r^2) / (n*512*8), e = D - target, r = stencil16(e on the 16^3 frame), the
+    residual the error leaves in the guess (the stencil is symmetric, so dL/de gets 2 res_weight stencil16(r))."""
     D, c = forward(p, x0, bf16)
     n = D.shape[0]
     t = np.asarray(target, np.float64).reshape(n, 512, NOUT)
@@ -163,6 +175,10 @@ def loss_grad(p, x0, target, bf16=True):
     loss = float(np.mean(e * e))
     r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
     dD = 2.0 * e / e.size
+    if res_weight:
+        res = stencil16(depth_to_space(e))
+        loss += float(res_weight * np.sum(res * res) / e.size)
+        dD = dD + interior_to_depth(2.0 * res_weight * stencil16(res) / e.size)
     dh = np.zeros((n, 8, 8, 8, 16))
     dh[..., :NOUT] = r(dD).reshape(n, 8, 8, 8, NOUT)
     g = np.zeros(NPARAMS)

@@ -45,6 +45,43 @@ def test_backward_matches_finite_differences(seed):
     assert np.all(g[K.OW[6] + K.NOUT * 432:K.OB[6]] == 0) and np.all(g[K.OB[6] + K.NOUT:K.OB[6] + 16] == 0)
 
 
+@pytest.mark.parametrize("seed", [5, 6])
+def test_residual_loss_backward_matches_finite_differences(seed):
+    p, x0, t = _random_problem(seed)
+    w = 0.05
+    _, g = K.loss_grad(p, x0, t, bf16=False, res_weight=w)
+    _, g0 = K.loss_grad(p, x0, t, bf16=False)
+    assert np.abs(g - g0).max() > 1e-3 * np.abs(g0).max()    # the residual term changes the gradient
+    probes = [K.OW[6] + 2 * 432 + 40, K.OB[6] + 7, K.OW[1] + 9 * 432 + 100, K.OB[3] + 4, K.OW[4] + 3 * 432 + 5]
+    eps = 1e-6
+    for i in probes:
+        e = np.zeros(K.NPARAMS)
+        e[i] = eps
+        lp, _ = K.loss_grad(p + e, x0, t, bf16=False, res_weight=w)
+        lm, _ = K.loss_grad(p - e, x0, t, bf16=False, res_weight=w)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, fd, g[i])
+
+
+def test_residual_stencil():
+    # a unit spike: -6 at the cell, 1 on each neighbour inside the frame (a face cell loses the outside one)
+    e = np.zeros((1, 16, 16, 16))
+    e[0, 5, 6, 7] = 1.0
+    e[0, 0, 9, 9] = 2.0
+    r = K.stencil16(e)
+    assert r[0, 5, 6, 7] == -6 and r[0, 4, 6, 7] == 1 and r[0, 6, 6, 7] == 1 and r[0, 5, 5, 7] == 1
+    assert r[0, 5, 7, 7] == 1 and r[0, 5, 6, 6] == 1 and r[0, 5, 6, 8] == 1
+    assert r[0, 0, 9, 9] == -12 and r[0, 1, 9, 9] == 2 and np.sum(r[0, :, 9, 9] != 0) == 2
+    # quadratic x^2 + y^2 + z^2: 6 wherever no neighbour falls outside
+    ax = np.arange(16.0)
+    q = (ax[:, None, None] ** 2 + ax[None, :, None] ** 2 + ax[None, None, :] ** 2)[None]
+    assert np.all(K.stencil16(q)[0, 1:-1, 1:-1, 1:-1] == 6.0)
+    # symmetric (self-adjoint with the zero exterior), which the gradient 2 w stencil16(r) relies on
+    rng = np.random.default_rng(7)
+    a, b = rng.standard_normal((2, 1, 16, 16, 16))
+    assert abs(np.sum(K.stencil16(a) * b) - np.sum(a * K.stencil16(b))) < 1e-9 * np.sum(np.abs(a))
+
+
 def test_zero_output_layer_gives_zero_correction():
     p, x0, _ = _random_problem(3, n=2)
     p[K.OW[6]:] = 0.0

@@ -114,7 +114,8 @@ def test_forward_matches_oracle(ctx, n):
         assert _rel(got[k], want[k]) < TOL_NET
 
 
-def test_loss_and_gradient_match_oracle(ctx):
+@pytest.mark.parametrize("res_weight", [0.0, 0.05])
+def test_loss_and_gradient_match_oracle(ctx, res_weight):
     torch = _torch()
     n = 2
     p = _random_params(7)
@@ -123,14 +124,17 @@ def test_loss_and_gradient_match_oracle(ctx):
     net = pg.CNN(ctx)
     try:
         net.set_params(p)
+        net.set_loss(res_weight)
         d_g = torch.empty(CO.NPARAMS, device="cuda")
-        print("[case] loss_grad n=2", flush=True)
+        print(f"[case] loss_grad n=2 res_weight={res_weight}", flush=True)
         loss = net.loss_grad(_dev(bits.view(np.int16)), _dev(tgt), n, d_g)
         torch.cuda.synchronize()
         g = d_g.cpu().numpy()
+        with pytest.raises(pg.PgError):
+            net.set_loss(-1.0)
     finally:
         net.close()
-    loss_o, g_o = CO.loss_grad(p, x0, tgt, bf16=True)
+    loss_o, g_o = CO.loss_grad(p, x0, tgt, bf16=True, res_weight=res_weight)
     assert abs(loss - loss_o) <= 1e-4 * loss_o
     blocks = [(f"W{l}", CO.OW[l], 16 * 432) for l in range(1, 7)] + [(f"b{l}", CO.OB[l], 16) for l in range(1, 7)]
     errs = {}
