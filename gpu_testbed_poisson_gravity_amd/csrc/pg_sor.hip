@@ -73,21 +73,55 @@ __device__ __forceinline__ bool wave_any(bool pred) { return __ballot(pred) != 0
 
 // ---- interpolation into LDS --------------------------------------------------------------------------
 
+// Coarse cell q (1-based i, j, k of the NC^3 block that has children in the cube) of the interpolation
+// loop: every cell, or (SHELL) only the 2 NC^2 + 4 (NC-1)(NC-2) cells on the outer layer of the block, whose
+// children include every cell of the cube's Dirichlet layer (the only cells a caller's guess leaves to the
+// interpolation).
+template <int NC, bool SHELL>
+__device__ __forceinline__ void coarse_cell(int q, int& i, int& j, int& k) {
+  if (!SHELL) {
+    i = q % NC + 1;
+    j = (q / NC) % NC + 1;
+    k = q / (NC * NC) + 1;
+    return;
+  }
+  constexpr int F = NC * NC, RING = 4 * (NC - 1);
+  if (q < 2 * F) {   // the bottom and top faces
+    const int t = q % F;
+    k = q < F ? 1 : NC;
+    i = t % NC + 1;
+    j = t / NC + 1;
+    return;
+  }
+  const int r = q - 2 * F, t = r % RING;   // the ring of each layer in between
+  k = r / RING + 2;
+  if (t < 2 * NC) {
+    j = t < NC ? 1 : NC;
+    i = t % NC + 1;
+  } else {
+    const int u = t - 2 * NC;
+    i = u < NC - 2 ? 1 : NC;
+    j = u % (NC - 2) + 2;
+  }
+}
+
 // Interpolate a patch's coarse potential into its LDS cube (split layout).  `lane`/`nlanes` index the
 // lanes serving this patch.  Two coarse cells per lane and step: both 27-value neighbourhoods are
-// requested before either is interpolated, halving the memory round trips of the phase.
-template <int GS>
+// requested before either is interpolated, halving the memory round trips of the phase.  SHELL: only the
+// cells that reach the Dirichlet layer (init from a caller's guess, which overwrites the interior).
+template <int GS, bool SHELL>
 __device__ void interp_to_lds(const float* __restrict__ cp, float* __restrict__ lds, int lane, int nlanes,
                               int int_scheme) {
   using S = SorGeo<GS>;
   constexpr int C = S::C, N = S::N, U = S::U, HP = S::HP, NC = C - 2;
-  constexpr int NCELL = NC * NC * NC;
+  constexpr int NCELL = SHELL ? 2 * NC * NC + 4 * (NC - 1) * (NC - 2) : NC * NC * NC;
   for (int q0 = lane; q0 < NCELL; q0 += 2 * nlanes) {
     float v[2][3][3][3];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       const int q = q0 + u * nlanes < NCELL ? q0 + u * nlanes : q0;
-      const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
+      int i, j, k;
+      coarse_cell<NC, SHELL>(q, i, j, k);
 #pragma unroll
       for (int dk = 0; dk < 3; dk++)
 #pragma unroll
@@ -100,7 +134,8 @@ __device__ void interp_to_lds(const float* __restrict__ cp, float* __restrict__ 
     for (int u = 0; u < 2; u++) {
       const int q = q0 + u * nlanes;
       if (q >= NCELL) break;
-      const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
+      int i, j, k;
+      coarse_cell<NC, SHELL>(q, i, j, k);
       float ch[2][2][2];
       interp_children(v[u], int_scheme, ch);
 #pragma unroll
@@ -309,8 +344,14 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
   const int m = plc % MX, yb = plc / MX;
   const int lb = 2 * yb * HP + m;
 
-  // A. coarse -> fine interpolation of the whole cube (initial guess and Dirichlet layer)
-  if (lp.valid) interp_to_lds<GS>(lp.cpot, lds, pl, SEG, P.int_scheme);
+  // A. coarse -> fine interpolation of the whole cube (initial guess and Dirichlet layer); with a caller's
+  // guess only the Dirichlet layer is needed (a rejected guess restarts from zero, not from the interpolation)
+  if (lp.valid) {
+    if (P.init_mode == 2)
+      interp_to_lds<GS, true>(lp.cpot, lds, pl, SEG, P.int_scheme);
+    else
+      interp_to_lds<GS, false>(lp.cpot, lds, pl, SEG, P.int_scheme);
+  }
   __syncthreads();
 
   // B. owned cells into registers
