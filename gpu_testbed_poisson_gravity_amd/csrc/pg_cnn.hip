@@ -652,10 +652,12 @@ __device__ __forceinline__ void zero_border(uint4* buf) {
 }
 
 // Interpolated solve cube from a coarse cube already staged in LDS (same arithmetic as interp_cube).
+// Returns this thread's max |value| over the cells it wrote (the cube's max once reduced over the workgroup).
 template <int GS>
-__device__ void interp_cube_lds(const float* cs, int int_scheme, float* cube) {
+__device__ float interp_cube_lds(const float* cs, int int_scheme, float* cube) {
   constexpr Geo g = geo_of(GS);
   constexpr int N = g.N, C = g.C, U = g.U, NC = C - 2;
+  float mx = 0.0f;
   for (int q = threadIdx.x; q < NC * NC * NC; q += kFT) {
     const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
     float v[3][3][3];
@@ -668,10 +670,13 @@ __device__ void interp_cube_lds(const float* cs, int int_scheme, float* cube) {
       for (int cy = 0; cy < 2; cy++)
         for (int cx = 0; cx < 2; cx++) {
           const int X = 2 * (i - 1) + cx - U, Y = 2 * (j - 1) + cy - U, Z = 2 * (k - 1) + cz - U;
-          if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) cube[(Z * N + Y) * N + X] = ch[cz][cy][cx];
+          if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) {
+            cube[(Z * N + Y) * N + X] = ch[cz][cy][cx];
+            mx = fmaxf(mx, fabsf(ch[cz][cy][cx]));
+          }
         }
   }
-  __syncthreads();
+  return mx;
 }
 
 template <int GS>
@@ -701,33 +706,21 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
     rv[k] = i4 < R3 / 4 ? reinterpret_cast<const float4*>(rho)[i4] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
-  interp_cube_lds<GS>(L.c.cs, int_scheme, cube);   // ends with a barrier: the coarse staging is dead
-  float mx = 0.0f;
-  for (int t = threadIdx.x; t < N * N * N; t += kFT) mx = fmaxf(mx, fabsf(cube[t]));
+  float mx = interp_cube_lds<GS>(L.c.cs, int_scheme, cube);
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
   if (lane == 0) L.red[wave] = mx;
-  __syncthreads();
+  __syncthreads();   // the cube is complete and the coarse staging is dead
   mx = L.red[0];
 #pragma unroll
   for (int w = 1; w < kFT / 64; w++) mx = fmaxf(mx, L.red[w]);
   const float s = mx > 0.0f ? mx : 1.0f;
   const float inv = 1.0f / s;
-  // packed input, pass 1: interpolated potential (low halves), zero density (high halves)
+  // packed input: channel pair (q0, q1) of sub-cell d of voxel vox is one 32-bit word (q0 low) at
+  // half-plane d/4, in-half channels 2(d%4), 2(d%4)+1.  Solve-interior cells get q0 = interp/s and
+  // q1 = 64 r/s (r from the cube, still in B, and the density read above); the other frame cells get
+  // q0 = interp/s on the cube's Dirichlet layer, 0 outside the cube, and q1 = 0.  Both loops write disjoint
+  // words, so they share one barrier.
   uint32_t* A32 = reinterpret_cast<uint32_t*>(A);
-  for (int vox = threadIdx.x; vox < kCnnPadVox; vox += kFT) {
-    const int cx = vox % S, cy = (vox / S) % S, cz = vox / (S * S);
-#pragma unroll
-    for (int d = 0; d < 8; d++) {
-      const int X = 2 * cx - 1 + (d & 1) - OFF, Y = 2 * cy - 1 + ((d >> 1) & 1) - OFF, Z = 2 * cz - 1 + (d >> 2) - OFF;
-      float c0 = 0.0f;
-      if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) c0 = cube[(Z * N + Y) * N + X] * inv;
-      // channels 2d, 2d+1 live in half-plane d/4 at in-half channels 2(d%4), 2(d%4)+1
-      A32[(d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)] = (uint32_t)f2bf(c0);
-    }
-  }
-  __syncthreads();
-  // pass 2: 64 r/s into the high halves (interior cells only; r from the cube, still in B, and the density
-  // read above)
 #pragma unroll
   for (int k = 0; k < RV; k++) {
     const int i4 = threadIdx.x + k * kFT;
@@ -741,10 +734,24 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
         const int ex = lx + 2 + OFF, ey = ly + 2 + OFF, ez = lz + 2 + OFF;
         const int vox = ((ez >> 1) * S + (ey >> 1)) * S + (ex >> 1);
         const int d = ((ez & 1) * 2 + (ey & 1)) * 2 + (ex & 1);
+        const float c = cube[((lz + 1) * N + ly + 1) * N + lx + 1];
         const float r = interp_residual<N>(cube, lx + 1, ly + 1, lz + 1, cst * e4[e]);
-        A[((d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)) * 2 + 1] = f2bf((r * inv) * kResScale);
-        L.c.in[idx] = cube[((lz + 1) * N + ly + 1) * N + lx + 1];
+        A32[(d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)] =
+            (uint32_t)f2bf(c * inv) | ((uint32_t)f2bf((r * inv) * kResScale) << 16);
+        L.c.in[idx] = c;
       }
+    }
+  }
+  for (int vox = threadIdx.x; vox < kCnnPadVox; vox += kFT) {
+    const int cx = vox % S, cy = (vox / S) % S, cz = vox / (S * S);
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+      const int X = 2 * cx - 1 + (d & 1) - OFF, Y = 2 * cy - 1 + ((d >> 1) & 1) - OFF, Z = 2 * cz - 1 + (d >> 2) - OFF;
+      const bool interior = X >= 1 && X < N - 1 && Y >= 1 && Y < N - 1 && Z >= 1 && Z < N - 1;
+      if (interior) continue;
+      float c0 = 0.0f;
+      if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) c0 = cube[(Z * N + Y) * N + X] * inv;
+      A32[(d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)] = (uint32_t)f2bf(c0);
     }
   }
   __syncthreads();   // every lane is done reading the cube in B
