@@ -40,6 +40,7 @@ class Options(C.Structure):
         ("rel_tol", C.c_float),
         ("fuse_gravity", C.c_int),
         ("guess_fallback_tol", C.c_float),
+        ("mixed_persistent", C.c_int),
     ]
 
 

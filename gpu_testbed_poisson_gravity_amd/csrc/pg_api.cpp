@@ -68,6 +68,7 @@ int validate_options(const pgamd_options& o) {
   if (o.check_every < 1) return fail(PGAMD_ERR_ARG, "check_every %d < 1", o.check_every);
   if (!(o.rel_tol >= 0.0f)) return fail(PGAMD_ERR_ARG, "rel_tol must be >= 0");
   if (!(o.guess_fallback_tol >= 0.0f)) return fail(PGAMD_ERR_ARG, "guess_fallback_tol must be >= 0");
+  if (o.mixed_persistent != 0 && o.mixed_persistent != 1) return fail(PGAMD_ERR_ARG, "mixed_persistent must be 0 or 1");
   return PGAMD_OK;
 }
 
@@ -179,6 +180,7 @@ void pgamd_default_options(pgamd_options* o) {
   o->rel_tol = 1e-5f;
   o->fuse_gravity = 1;
   o->guess_fallback_tol = 0.0f;
+  o->mixed_persistent = 0;
 }
 
 const char* pgamd_last_error(void) { return g_last_error.c_str(); }

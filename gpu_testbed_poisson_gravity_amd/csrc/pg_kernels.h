@@ -115,6 +115,11 @@ hipError_t launch_sor_persistent(const MixedBatch& b, const ParamsByGs& p, int p
                                  hipStream_t stream);
 hipError_t launch_mg_persistent(const MixedBatch& b, const ParamsByGs& p, int grid, size_t lds, hipStream_t stream);
 int sor_persistent_blocks_per_cu(int ppw1, int ppw2, size_t lds);
+// Per-ghost-size launches of a mixed batch (the default): the SOR tasks [first, first + count) of ghost size
+// gs (ppw patches per wavefront, the persistent kernel's layouts), and the MG tasks likewise.
+hipError_t launch_sor_tasks(const MixedBatch& b, const SolveParams& p, int gs, int ppw, int first, int count,
+                            hipStream_t stream);
+hipError_t launch_mg_tasks(const MixedBatch& b, const SolveParams& p, int gs, int first, int count, hipStream_t stream);
 int mg_persistent_blocks_per_cu(size_t lds);
 // Zeroes n work-queue counters with a kernel (graph-capture safe, no memset node).
 hipError_t launch_reset_counters(unsigned int* counters, int n, hipStream_t stream);

@@ -455,6 +455,35 @@ __global__ void __launch_bounds__(kMgThreads) mg_persistent_kernel(MixedBatch b,
   }
 }
 
+// The MG tasks [first, first + gridDim.x) of one ghost size, one patch per workgroup (no task loop: the
+// patch solve compiles as in the uniform kernel, without the persistent kernel's spills).
+template <int GS>
+__global__ void __launch_bounds__(kMgThreads) mg_tasks_kernel(MixedBatch b, int first, SolveParams Q) {
+  extern __shared__ float lds[];
+  const Task tk = b.tasks[first + blockIdx.x];
+  const int patch = b.perm[tk.first];
+  if (patch < 0 || patch >= b.npatch || tk.gs != GS) return;   // malformed plan: skip (uniform per workgroup)
+  const PatchDesc d = b.desc[patch];
+  const int G = gra_nxt(Q.gra_ghost);
+  mg_patch<GS>(lds, b.rho + d.rho_off, b.cpot + d.cpot_off, b.guess ? b.guess + d.rho_off : nullptr,
+               b.pot_out + (size_t)patch * G * G * G, b.flu ? b.flu + (size_t)patch * kNComp * kPatchCells : nullptr,
+               b.iters ? b.iters + patch : nullptr, b.status ? b.status + patch : nullptr, Q);
+}
+
+hipError_t launch_mg_tasks(const MixedBatch& b, const SolveParams& p, int gs, int first, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const dim3 grid(count), block(kMgThreads);
+  switch (gs) {
+    case 1: hipLaunchKernelGGL((mg_tasks_kernel<1>), grid, block, lds_bytes_mg(1), s, b, first, p); break;
+    case 2: hipLaunchKernelGGL((mg_tasks_kernel<2>), grid, block, lds_bytes_mg(2), s, b, first, p); break;
+    case 3: hipLaunchKernelGGL((mg_tasks_kernel<3>), grid, block, lds_bytes_mg(3), s, b, first, p); break;
+    case 4: hipLaunchKernelGGL((mg_tasks_kernel<4>), grid, block, lds_bytes_mg(4), s, b, first, p); break;
+    case 5: hipLaunchKernelGGL((mg_tasks_kernel<5>), grid, block, lds_bytes_mg(5), s, b, first, p); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 int mg_persistent_blocks_per_cu(size_t lds) {
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&mg_persistent_kernel),

@@ -33,6 +33,9 @@ struct pgamd_plan {
   Task* d_tasks = nullptr;        // SOR tasks then MG tasks
   unsigned int* d_counters = nullptr;  // [0] SOR queue, [1] MG queue
   bool has_gs[6] = {false, false, false, false, false, false};
+  // per ghost size: first task and task count (SOR tasks index d_tasks, MG tasks index d_tasks + n_sor_tasks)
+  // and the SOR patches per wavefront, for the per-ghost-size launches
+  int sor_first[6] = {}, sor_count[6] = {}, sor_ppw[6] = {}, mg_first[6] = {}, mg_count[6] = {};
 };
 
 struct pgamd_graph {
@@ -86,7 +89,8 @@ int enqueue_mixed(pgamd_context* c, pgamd_plan* pl, const float* rho, const floa
   b.flu = fused ? flu : nullptr;
   b.iters = iters;
   b.status = status;
-  hipError_t e = launch_reset_counters(pl->d_counters, 2, st);
+  const bool persistent = o.mixed_persistent != 0;
+  hipError_t e = persistent ? launch_reset_counters(pl->d_counters, 2, st) : hipSuccess;
   if (e != hipSuccess) return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e));
   if (a.poisson) {
     hipStream_t aux = ctx_aux_stream(c);
@@ -102,8 +106,13 @@ int enqueue_mixed(pgamd_context* c, pgamd_plan* pl, const float* rho, const floa
       bm.tasks = pl->d_tasks + pl->n_sor_tasks;
       bm.ntasks = pl->n_mg_tasks;
       bm.counter = pl->d_counters + 1;
-      if ((e = launch_mg_persistent(bm, P, pl->grid_mg, pl->lds_mg, aux)) != hipSuccess)
-        return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e));
+      if (persistent) {
+        e = launch_mg_persistent(bm, P, pl->grid_mg, pl->lds_mg, aux);
+      } else {
+        for (int g = kMaxGhost; g >= 1 && e == hipSuccess; g--)
+          e = launch_mg_tasks(bm, P.p[g], g, pl->mg_first[g], pl->mg_count[g], aux);
+      }
+      if (e != hipSuccess) return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e));
       if ((e = hipEventRecord(ev_join, aux)) != hipSuccess) return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e));
     }
     MixedBatch bs = b;
@@ -111,8 +120,13 @@ int enqueue_mixed(pgamd_context* c, pgamd_plan* pl, const float* rho, const floa
     bs.tasks = pl->d_tasks;
     bs.ntasks = pl->n_sor_tasks;
     bs.counter = pl->d_counters;
-    if ((e = launch_sor_persistent(bs, P, pl->ppw1, pl->ppw2, pl->grid_sor, pl->lds_sor, st)) != hipSuccess)
-      return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e));
+    if (persistent) {
+      e = launch_sor_persistent(bs, P, pl->ppw1, pl->ppw2, pl->grid_sor, pl->lds_sor, st);
+    } else {
+      for (int g = kMaxGhost; g >= 1 && e == hipSuccess; g--)
+        e = launch_sor_tasks(bs, P.p[g], g, pl->sor_ppw[g], pl->sor_first[g], pl->sor_count[g], st);
+    }
+    if (e != hipSuccess) return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e));
     if (pl->n_mg_tasks > 0 && (e = hipStreamWaitEvent(st, ev_join, 0)) != hipSuccess)
       return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e));
   }
@@ -192,15 +206,20 @@ int pgamd_plan_create(pgamd_context* c, const int* ghost_size, const int* scheme
   for (int g = kMaxGhost; g >= 1; g--) {
     const int ppw = g == 1 ? pl->ppw1 : (g == 2 ? pl->ppw2 : 1);
     const auto& v = sor_by_gs[g];
+    pl->sor_first[g] = (int)tasks.size();
+    pl->sor_ppw[g] = ppw;
     for (size_t i = 0; i < v.size(); i += ppw) {
       const int cnt = (int)std::min<size_t>(ppw, v.size() - i);
       tasks.push_back(Task{g, PGAMD_SOR, (int)perm.size(), cnt});
       for (int k = 0; k < cnt; k++) perm.push_back(v[i + k]);
     }
+    pl->sor_count[g] = (int)tasks.size() - pl->sor_first[g];
     if (!v.empty()) lds_sor = std::max(lds_sor, lds_bytes_sor(g, ppw, g == 4 ? 2 : 0));
   }
   pl->n_sor_tasks = (int)tasks.size();
   for (int g = kMaxGhost; g >= 1; g--) {
+    pl->mg_first[g] = (int)tasks.size() - pl->n_sor_tasks;
+    pl->mg_count[g] = (int)mg_by_gs[g].size();
     for (int p : mg_by_gs[g]) {
       tasks.push_back(Task{g, PGAMD_MG, (int)perm.size(), 1});
       perm.push_back(p);

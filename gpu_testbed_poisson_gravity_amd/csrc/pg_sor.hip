@@ -110,7 +110,7 @@ __device__ __forceinline__ void coarse_cell(int q, int& i, int& j, int& k) {
 // requested before either is interpolated, halving the memory round trips of the phase.  SHELL: only the
 // cells that reach the Dirichlet layer (init from a caller's guess, which overwrites the interior).
 template <int GS, bool SHELL>
-__device__ void interp_to_lds(const float* __restrict__ cp, float* __restrict__ lds, int lane, int nlanes,
+__device__ __forceinline__ void interp_to_lds(const float* __restrict__ cp, float* __restrict__ lds, int lane, int nlanes,
                               int int_scheme) {
   using S = SorGeo<GS>;
   constexpr int C = S::C, N = S::N, U = S::U, HP = S::HP, NC = C - 2;
@@ -336,7 +336,10 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
   constexpr int N = S::N, R = S::R, HP = S::HP, MX = S::MX, LP = S::LP;
   constexpr int SEG = 64 / PPW;
   static_assert(LP <= SEG, "patch does not fit its lane segment");
-  const int lane = threadIdx.x;
+  int lane = threadIdx.x;
+  // opaque per patch: inside the persistent kernel's task loop the compiler would otherwise hoist every
+  // lane-derived address out of the loop and keep them all live (measured: 2 KB of spills per lane)
+  asm volatile("" : "+v"(lane));
   const int slot = lane / SEG, pl = lane % SEG;
   float* lds = lds_wave + slot * S::FLOATS;
   const bool active = lp.valid && pl < LP;
@@ -619,6 +622,54 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
       default: mixed_sor_task<layout_code(5, 0), 1>(lds, b, tk, P.p[5]); break;
     }
   }
+}
+
+// The SOR tasks [first, first + gridDim.x) of one ghost size, one task per wavefront-sized workgroup (the
+// plan lists each ghost size's tasks contiguously).  Without the persistent kernel's task loop the patch
+// solve compiles as in the uniform kernel, with no spills, and the hardware refills a CU's wave slots as
+// patches converge, which balances the ragged per-patch iteration counts without a queue.
+template <int L, int PPW>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    sor_tasks_kernel(MixedBatch b, int first, int gs, SolveParams P) {
+  extern __shared__ float lds[];
+  const Task tk = b.tasks[first + blockIdx.x];
+  if (tk.gs != gs) return;   // a malformed plan never runs a task with another ghost size's layout
+  mixed_sor_task<L, PPW>(lds, b, tk, P);
+}
+
+template <int L, int PPW>
+static void launch_sor_tasks_as(const MixedBatch& b, const SolveParams& p, int gs, int first, int count, size_t lds,
+                                hipStream_t s) {
+  hipLaunchKernelGGL((sor_tasks_kernel<L, PPW>), dim3(count), dim3(64), lds, s, b, first, gs, p);
+}
+
+hipError_t launch_sor_tasks(const MixedBatch& b, const SolveParams& p, int gs, int ppw, int first, int count,
+                            hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  // the layouts of the persistent kernel, so both launch modes give identical results
+  switch (gs) {
+    case 1:
+      if (ppw == 4)
+        launch_sor_tasks_as<layout_code(1, 0), 4>(b, p, gs, first, count, lds_bytes_sor(1, 4, 0), s);
+      else if (ppw == 2)
+        launch_sor_tasks_as<layout_code(1, 0), 2>(b, p, gs, first, count, lds_bytes_sor(1, 2, 0), s);
+      else
+        return hipErrorInvalidValue;
+      break;
+    case 2:
+      if (ppw == 2)
+        launch_sor_tasks_as<layout_code(2, 0), 2>(b, p, gs, first, count, lds_bytes_sor(2, 2, 0), s);
+      else if (ppw == 1)
+        launch_sor_tasks_as<layout_code(2, 0), 1>(b, p, gs, first, count, lds_bytes_sor(2, 1, 0), s);
+      else
+        return hipErrorInvalidValue;
+      break;
+    case 3: launch_sor_tasks_as<layout_code(3, 0), 1>(b, p, gs, first, count, lds_bytes_sor(3, 1, 0), s); break;
+    case 4: launch_sor_tasks_as<layout_code(4, 2), 1>(b, p, gs, first, count, lds_bytes_sor(4, 1, 2), s); break;
+    case 5: launch_sor_tasks_as<layout_code(5, 0), 1>(b, p, gs, first, count, lds_bytes_sor(5, 1, 0), s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 __global__ void reset_counters_kernel(unsigned int* c, int n) {

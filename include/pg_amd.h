@@ -71,10 +71,13 @@ typedef struct {
   int fuse_gravity;     /* 1: gravity update inside the solve kernel, phi still in LDS */
   float guess_fallback_tol; /* init GUESS, SOR: use the zero guess instead when the guess's relative
                                residual exceeds this (0: never) */
+  int mixed_persistent; /* mixed batches (pgamd_solve_mixed_device, graphs): 0 = one launch per ghost size and
+                           scheme, one task per workgroup; 1 = one persistent work-queue launch per scheme.
+                           Same results; 0 is faster on gfx950 (the persistent kernels spill registers) */
 } pgamd_options;
 
 /* Fills the defaults: ghost 5/1, SOR, interpolated guess, reference rule, check_every 1, rel_tol 1e-5,
- * fused gravity. */
+ * fused gravity, per-ghost-size launches for mixed batches. */
 void pgamd_default_options(pgamd_options* opt);
 
 /* Replaces CUAPI_Set_Diagnose_Device (GPU_API/CUAPI_Set_Diagnose_Device.cu:45-292): selects the device

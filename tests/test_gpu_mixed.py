@@ -1,4 +1,5 @@
-"""Mixed ghost sizes and schemes in one persistent launch, and HIP graph replay (needs an MI355X).
+"""Mixed ghost sizes and schemes (per-ghost-size launches, and one persistent work-queue launch per scheme),
+and HIP graph replay (needs an MI355X).
 
 Each patch of a mixed batch must give exactly what the oracle gives for that patch alone (fixed iteration
 counts: bit for bit), whatever the other patches are; graph replays must reproduce the eager solve bit for
@@ -70,13 +71,16 @@ def _run_mixed(ctx, plan, rho, cpot, flu, args, omega):
     return d_out.cpu().numpy(), d_flu.cpu().numpy(), d_it.cpu().numpy(), d_st.cpu().numpy()
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("ppw1", [4, 2])
-def test_mixed_fixed_iterations_bit_exact(ppw1):
+def test_mixed_fixed_iterations_bit_exact(ppw1, persistent):
     gs, sc, rho, cpot, flu = _batch(60, seed=1)
-    print(f"[case] mixed fixed ppw1={ppw1} gs counts {np.bincount(gs)} mg={int((sc == pg.MG).sum())}", flush=True)
+    print(f"[case] mixed fixed ppw1={ppw1} persistent={persistent} gs counts {np.bincount(gs)} "
+          f"mg={int((sc == pg.MG).sum())}", flush=True)
     kw = dict(sor_min_iter=30, sor_max_iter=30, mg_max_iter=3, mg_tol=0.0, mg_npre=2, mg_npost=2, dt=0.3)
     with pg.Context(0) as ctx:
-        ctx.set_options(init_mode=pg.INIT_INTERP, term_mode=pg.TERM_REFERENCE, fuse_gravity=1, gra_ghost_size=1)
+        ctx.set_options(init_mode=pg.INIT_INTERP, term_mode=pg.TERM_REFERENCE, fuse_gravity=1, gra_ghost_size=1,
+                        mixed_persistent=persistent)
         ctx.set_layout(1, ppw1, 0)
         ctx.set_layout(2, 2 if ppw1 == 4 else 1, 0)
         plan = ctx.plan(gs, sc)
@@ -110,13 +114,16 @@ def test_mixed_relres_zero_guess():
     assert err <= 1e-5
 
 
-def test_graph_replay_mixed_matches_eager():
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_graph_replay_mixed_matches_eager(persistent):
     import torch
 
     gs, sc, rho, cpot, flu = _batch(48, seed=3)
     n = len(gs)
+    print(f"[case] graph replay mixed persistent={persistent}", flush=True)
     with pg.Context(0) as ctx:
-        ctx.set_options(init_mode=pg.INIT_ZERO, term_mode=pg.TERM_RELRES, check_every=1, rel_tol=1e-5)
+        ctx.set_options(init_mode=pg.INIT_ZERO, term_mode=pg.TERM_RELRES, check_every=1, rel_tol=1e-5,
+                        mixed_persistent=persistent)
         plan = ctx.plan(gs, sc)
         args = pg.solver_args(poi_coeff=POI, sor_min_iter=10, sor_max_iter=1000, sor_omega=1.8, mg_max_iter=20,
                               mg_tol=1e-5, mg_npre=2, mg_npost=2, dt=0.1)
