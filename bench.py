@@ -75,7 +75,7 @@ def parse():
                          "0: serial only (measured faster on MI355X: the two kernels do not share CUs well)")
     ap.add_argument("--workload", choices=["config2", "config3", "config5"], default="config2",
                     help="config2 (default): the metric's workload; config3: 32768 mixed-GS lognormal patches, "
-                         "half SOR half MG, one graph-captured persistent launch; config5: 65536 mixed-GS "
+                         "half SOR half MG, one captured graph; config5: 65536 mixed-GS "
                          "patches (power law / Plummer / lognormal), warm start with guess check, SOR+MG, gravity")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size reference for "
@@ -87,7 +87,7 @@ OMEGA_BY_GS = [1.49, 1.57, 1.62, 1.65, 1.69]
 
 
 def run_mixed(a):
-    """Mixed ghost sizes and schemes in one persistent launch per scheme (configurations 3 and 5)."""
+    """Mixed ghost sizes and schemes, one captured graph of per-ghost-size launches (configurations 3 and 5)."""
     import torch
 
     import gpu_testbed_poisson_gravity_amd as pg
@@ -115,7 +115,8 @@ def run_mixed(a):
         init = pg.INIT_GUESS
     ctx.set_options(pot_ghost_size=5, gra_ghost_size=1, init_mode=init, term_mode=pg.TERM_RELRES, rel_tol=1e-5,
                     check_every=a.check_every, fuse_gravity=1, guess_fallback_tol=a.fallback_tol if net else 0.0)
-    # autotune every ghost size on its own SOR patches (layout table used by the persistent kernel)
+    # autotune every ghost size on its own SOR patches (the mixed launches use the row padding per ghost size
+    # and the patches per wavefront of ghost size 1)
     tuned = {}
     for g in range(1, 6):
         idx = np.nonzero((gs == g) & (sch == pg.SOR))[0][:4096]
@@ -174,7 +175,7 @@ def run_mixed(a):
         "data": "synthetic",
         "config": {"workload": f"{n} patches, GS uniform in 1..5, densities {'/'.join(kinds)}, half SOR (omega by "
                                f"GS {OMEGA_BY_GS}) half MG (2+2 smoothing), rel. residual 1e-5, fused gravity, one "
-                               f"graph-captured persistent launch per scheme, autotuned layouts {tuned}; start: "
+                               f"captured graph (one launch per ghost size and scheme), autotuned layouts {tuned}; start: "
                                + ("learned warm start (guess check 1e-2)" if net is not None else "interpolation"),
                    "npatch": n},
         "network_ms": pre, "solve_ms": sol,

@@ -116,8 +116,8 @@ hipError_t launch_sor_persistent(const MixedBatch& b, const ParamsByGs& p, int p
 hipError_t launch_mg_persistent(const MixedBatch& b, const ParamsByGs& p, int grid, size_t lds, hipStream_t stream);
 int sor_persistent_blocks_per_cu(int ppw1, int ppw2, size_t lds);
 // Per-ghost-size launches of a mixed batch (the default): the SOR tasks [first, first + count) of ghost size
-// gs (ppw patches per wavefront, the persistent kernel's layouts), and the MG tasks likewise.
-hipError_t launch_sor_tasks(const MixedBatch& b, const SolveParams& p, int gs, int ppw, int first, int count,
+// gs (ppw patches per wavefront, LDS row padding pad), and the MG tasks likewise.
+hipError_t launch_sor_tasks(const MixedBatch& b, const SolveParams& p, int gs, int ppw, int pad, int first, int count,
                             hipStream_t stream);
 hipError_t launch_mg_tasks(const MixedBatch& b, const SolveParams& p, int gs, int first, int count, hipStream_t stream);
 int mg_persistent_blocks_per_cu(size_t lds);

@@ -124,7 +124,8 @@ int enqueue_mixed(pgamd_context* c, pgamd_plan* pl, const float* rho, const floa
       e = launch_sor_persistent(bs, P, pl->ppw1, pl->ppw2, pl->grid_sor, pl->lds_sor, st);
     } else {
       for (int g = kMaxGhost; g >= 1 && e == hipSuccess; g--)
-        e = launch_sor_tasks(bs, P.p[g], g, pl->sor_ppw[g], pl->sor_first[g], pl->sor_count[g], st);
+        e = launch_sor_tasks(bs, P.p[g], g, pl->sor_ppw[g], ctx_layout(c).pad[g], pl->sor_first[g],
+                             pl->sor_count[g], st);
     }
     if (e != hipSuccess) return set_error(PGAMD_ERR_DEVICE, hipGetErrorString(e));
     if (pl->n_mg_tasks > 0 && (e = hipStreamWaitEvent(st, ev_join, 0)) != hipSuccess)

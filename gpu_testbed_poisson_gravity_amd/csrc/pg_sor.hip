@@ -637,39 +637,41 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   mixed_sor_task<L, PPW>(lds, b, tk, P);
 }
 
-template <int L, int PPW>
-static void launch_sor_tasks_as(const MixedBatch& b, const SolveParams& p, int gs, int first, int count, size_t lds,
-                                hipStream_t s) {
-  hipLaunchKernelGGL((sor_tasks_kernel<L, PPW>), dim3(count), dim3(64), lds, s, b, first, gs, p);
+template <int GS, int PPW, int PAD>
+static hipError_t launch_sor_tasks_as(const MixedBatch& b, const SolveParams& p, int first, int count, hipStream_t s) {
+  hipLaunchKernelGGL((sor_tasks_kernel<layout_code(GS, PAD), PPW>), dim3(count), dim3(64), lds_bytes_sor(GS, PPW, PAD),
+                     s, b, first, GS, p);
+  return hipGetLastError();
 }
 
-hipError_t launch_sor_tasks(const MixedBatch& b, const SolveParams& p, int gs, int ppw, int first, int count,
-                            hipStream_t s) {
-  if (count <= 0) return hipSuccess;
-  // the layouts of the persistent kernel, so both launch modes give identical results
-  switch (gs) {
-    case 1:
-      if (ppw == 4)
-        launch_sor_tasks_as<layout_code(1, 0), 4>(b, p, gs, first, count, lds_bytes_sor(1, 4, 0), s);
-      else if (ppw == 2)
-        launch_sor_tasks_as<layout_code(1, 0), 2>(b, p, gs, first, count, lds_bytes_sor(1, 2, 0), s);
-      else
-        return hipErrorInvalidValue;
-      break;
-    case 2:
-      if (ppw == 2)
-        launch_sor_tasks_as<layout_code(2, 0), 2>(b, p, gs, first, count, lds_bytes_sor(2, 2, 0), s);
-      else if (ppw == 1)
-        launch_sor_tasks_as<layout_code(2, 0), 1>(b, p, gs, first, count, lds_bytes_sor(2, 1, 0), s);
-      else
-        return hipErrorInvalidValue;
-      break;
-    case 3: launch_sor_tasks_as<layout_code(3, 0), 1>(b, p, gs, first, count, lds_bytes_sor(3, 1, 0), s); break;
-    case 4: launch_sor_tasks_as<layout_code(4, 2), 1>(b, p, gs, first, count, lds_bytes_sor(4, 1, 2), s); break;
-    case 5: launch_sor_tasks_as<layout_code(5, 0), 1>(b, p, gs, first, count, lds_bytes_sor(5, 1, 0), s); break;
+template <int GS, int PPW>
+static hipError_t launch_sor_tasks_pad(const MixedBatch& b, const SolveParams& p, int pad, int first, int count,
+                                       hipStream_t s) {
+  switch (pad) {
+    case 0: return launch_sor_tasks_as<GS, PPW, 0>(b, p, first, count, s);
+    case 1: return launch_sor_tasks_as<GS, PPW, 1>(b, p, first, count, s);
+    case 2: return launch_sor_tasks_as<GS, PPW, 2>(b, p, first, count, s);
+    case 3: return launch_sor_tasks_as<GS, PPW, 3>(b, p, first, count, s);
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
+}
+
+// The same (gs, patches per wavefront, row padding) layouts as the uniform kernel; the padding only moves LDS
+// rows, so results do not depend on it.
+hipError_t launch_sor_tasks(const MixedBatch& b, const SolveParams& p, int gs, int ppw, int pad, int first, int count,
+                            hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  switch (gs * 8 + ppw) {
+    case 1 * 8 + 1: return launch_sor_tasks_pad<1, 1>(b, p, pad, first, count, s);
+    case 1 * 8 + 2: return launch_sor_tasks_pad<1, 2>(b, p, pad, first, count, s);
+    case 1 * 8 + 4: return launch_sor_tasks_pad<1, 4>(b, p, pad, first, count, s);
+    case 2 * 8 + 1: return launch_sor_tasks_pad<2, 1>(b, p, pad, first, count, s);
+    case 2 * 8 + 2: return launch_sor_tasks_pad<2, 2>(b, p, pad, first, count, s);
+    case 3 * 8 + 1: return launch_sor_tasks_pad<3, 1>(b, p, pad, first, count, s);
+    case 4 * 8 + 1: return launch_sor_tasks_pad<4, 1>(b, p, pad, first, count, s);
+    case 5 * 8 + 1: return launch_sor_tasks_pad<5, 1>(b, p, pad, first, count, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 __global__ void reset_counters_kernel(unsigned int* c, int n) {
