@@ -75,8 +75,8 @@ __device__ __forceinline__ bool wave_any(bool pred) { return __ballot(pred) != 0
 
 // Coarse cell q (1-based i, j, k of the NC^3 block that has children in the cube) of the interpolation
 // loop: every cell, or (SHELL) only the 2 NC^2 + 4 (NC-1)(NC-2) cells on the outer layer of the block, whose
-// children include every cell of the cube's Dirichlet layer (the only cells a caller's guess leaves to the
-// interpolation).
+// children include every cell of the cube's Dirichlet layer (the only cells a zero start or a caller's guess
+// leaves to the interpolation).
 template <int NC, bool SHELL>
 __device__ __forceinline__ void coarse_cell(int q, int& i, int& j, int& k) {
   if (!SHELL) {
@@ -108,7 +108,7 @@ __device__ __forceinline__ void coarse_cell(int q, int& i, int& j, int& k) {
 // Interpolate a patch's coarse potential into its LDS cube (split layout).  `lane`/`nlanes` index the
 // lanes serving this patch.  Two coarse cells per lane and step: both 27-value neighbourhoods are
 // requested before either is interpolated, halving the memory round trips of the phase.  SHELL: only the
-// cells that reach the Dirichlet layer (init from a caller's guess, which overwrites the interior).
+// cells that reach the Dirichlet layer (zero start or a caller's guess: the interior is overwritten).
 template <int GS, bool SHELL>
 __device__ __forceinline__ void interp_to_lds(const float* __restrict__ cp, float* __restrict__ lds, int lane, int nlanes,
                               int int_scheme) {
@@ -347,10 +347,11 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
   const int m = plc % MX, yb = plc / MX;
   const int lb = 2 * yb * HP + m;
 
-  // A. coarse -> fine interpolation of the whole cube (initial guess and Dirichlet layer); with a caller's
-  // guess only the Dirichlet layer is needed (a rejected guess restarts from zero, not from the interpolation)
+  // A. coarse -> fine interpolation of the whole cube (initial guess and Dirichlet layer); a zero start or a
+  // caller's guess needs only the Dirichlet layer (a rejected guess restarts from zero, not from the
+  // interpolation)
   if (lp.valid) {
-    if (P.init_mode == 2)
+    if (P.init_mode != 0)
       interp_to_lds<GS, true>(lp.cpot, lds, pl, SEG, P.int_scheme);
     else
       interp_to_lds<GS, false>(lp.cpot, lds, pl, SEG, P.int_scheme);
