@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "patch-solves/s at rel. residual ≤1e-5: warm-start vs plain SOR, 1 MI355X"
 HBM_PEAK = 8.0e12          # B/s, MI355X published
-LDS_PEAK = 256 * 128 * 2.4e9  # B/s: 256 CUs x 128 B/clk x 2.4 GHz
+LDS_PEAK = 256 * 128 * 2.4e9  # B/s: 256 CUs x 128 B/clk x 2.4 GHz, used when the measurement is unavailable
 MFMA_BF16_PEAK = 2.5e15    # FLOP/s, dense BF16, MI355X published
 LDS_BYTES_PER_UPDATE = 36  # 7 phi reads + 1 rho read + 1 write, FP32
 # Warm-start network FLOPs per patch: six 3^3 convolutions on the 8^3 grid, 16 channels in, 16 out
@@ -50,6 +50,17 @@ def measured_traffic(a):
             if len(w) == 4 and w[0] == "traffic":
                 out[(w[1], w[2])] = float(w[3])
     return out
+
+
+def measure_peaks(ctx):
+    """LDS read and HBM copy bandwidth measured on the device (GB/s), or None if the entry fails."""
+    try:
+        lds, hbm = ctx.measure_peaks()
+    except Exception as e:  # the peaks only feed the roofline report
+        log(f"[bench] peak measurement failed: {e}")
+        return None
+    log(f"[bench] measured peaks: LDS read {lds / 1e12:.1f} TB/s, HBM copy {hbm / 1e12:.2f} TB/s")
+    return {"lds_read_GBps": lds / 1e9, "hbm_copy_GBps": hbm / 1e9}
 
 
 def parse():
@@ -107,6 +118,8 @@ def run_mixed(a):
                           mg_npost=2, mg_tol=1e-5, poi_coeff=poi)
     omega = np.asarray(OMEGA_BY_GS, np.float32)
     ctx = pg.Context(0)
+    peaks = measure_peaks(ctx)
+    lds_peak = peaks["lds_read_GBps"] * 1e9 if peaks else LDS_PEAK
     net = None
     init = pg.INIT_INTERP
     if c5 and os.path.exists(a.weights):
@@ -183,10 +196,11 @@ def run_mixed(a):
         "mg": {"mean_vcycles": float(it[sch == pg.MG].mean()), "max_vcycles": int(it[sch == pg.MG].max())},
         "converged": int(((st & pg.STATUS_CONVERGED) != 0).sum()),
         "fallbacks": int(((st & pg.STATUS_FALLBACK) != 0).sum()),
-        "roofline": {"bound": "lds", "achieved": sor_lds / (sol * 1e-3) / 1e9, "peak": LDS_PEAK / 1e9, "unit": "GB/s",
-                     "frac": sor_lds / (sol * 1e-3) / LDS_PEAK, "traffic": None,
+        "roofline": {"bound": "lds", "achieved": sor_lds / (sol * 1e-3) / 1e9, "peak": lds_peak / 1e9, "unit": "GB/s",
+                     "frac": sor_lds / (sol * 1e-3) / lds_peak, "traffic": None,
+                     "peak_basis": "measured LDS read bandwidth (pgamd_measure_peaks)" if peaks else "placeholder",
                      "note": "SOR patches' algorithmic LDS bytes over the whole graph time (MG time included)"},
-        "cpu_baseline": None,
+        "cpu_baseline": None, "peaks_measured": peaks,
     }
     print(json.dumps(out))
     graph.close()
@@ -226,6 +240,8 @@ def main():
     poi = 4.0 * np.pi * 1e-5
     args = pg.solver_args(dt=1e-3, sor_min_iter=10, sor_max_iter=1000, sor_omega=1.8, poi_coeff=poi)
     ctx = pg.Context(0)
+    peaks = measure_peaks(ctx)
+    lds_peak = peaks["lds_read_GBps"] * 1e9 if peaks else LDS_PEAK
     if a.pad is not None or a.ppw is not None:
         ppw0, pad0 = ctx.get_layout(a.gs)
         ctx.set_layout(a.gs, a.ppw if a.ppw is not None else ppw0, a.pad if a.pad is not None else pad0)
@@ -288,8 +304,10 @@ def main():
         alg = float(r["iters"].astype(np.float64).sum()) * R ** 3 * LDS_BYTES_PER_UPDATE
         ach = alg / (r["solve_ms"] * 1e-3)
         hbm = a.npatch * 4.0 * (R ** 3 * (2 if leg == "warm" else 1) + g["C"] ** 3 + 10 ** 3 + 4 * 512 + 5 * 512)
-        return {"kernel": "sor_uniform_kernel", "bound": "lds", "achieved": ach / 1e9, "peak": LDS_PEAK / 1e9,
-                "unit": "GB/s", "frac": ach / LDS_PEAK, "traffic": traffic.get(("sor_uniform_kernel", leg)),
+        return {"kernel": "sor_uniform_kernel", "bound": "lds", "achieved": ach / 1e9, "peak": lds_peak / 1e9,
+                "unit": "GB/s", "frac": ach / lds_peak, "traffic": traffic.get(("sor_uniform_kernel", leg)),
+                "peak_basis": "measured LDS read bandwidth (pgamd_measure_peaks)" if peaks else "placeholder",
+                "frac_of_placeholder": ach / LDS_PEAK,
                 "algorithmic_bytes_per_launch": alg, "launch_ms": r["solve_ms"],
                 "hbm_algorithmic_bytes_per_launch": hbm,
                 "hbm_achieved": hbm / (r["solve_ms"] * 1e-3) / 1e9, "hbm_peak": HBM_PEAK / 1e9}
@@ -366,7 +384,7 @@ def main():
         "plain": strip(plain), "warm": strip(warm),
         "warm_vs_plain": (warm["value"] / plain["value"]) if (warm is not None and plain is not None) else None,
         "wall_ms_per_step": head_r["wall_ms_per_step"],
-        "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu,
+        "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu, "peaks_measured": peaks,
     }
     print(json.dumps(out))
     if net is not None:

@@ -75,6 +75,7 @@ SIGNATURES = {
     "pgamd_set_options": (C.c_int, [_CTX, C.POINTER(Options)]),
     "pgamd_last_error": (C.c_char_p, []),
     "pgamd_device_info": (C.c_int, [_CTX, C.c_char_p, C.c_int]),
+    "pgamd_measure_peaks": (C.c_int, [_CTX, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pgamd_mem_allocate_poisson_gravity": (C.c_int, [_CTX, C.c_int, _FPP, _FPP, _FPP, _FPP]),
     "pgamd_mem_free_poisson_gravity": (C.c_int, [_CTX, _FPP, _FPP, _FPP, _FPP]),
     "pgamd_asyn_poisson_gravity_solver": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
@@ -219,6 +220,12 @@ class Context:
         buf = C.create_string_buffer(256)
         _check(load().pgamd_device_info(self.h, buf, 256))
         return buf.value.decode()
+
+    def measure_peaks(self):
+        """(LDS read B/s, HBM copy B/s) measured on this context's device (pgamd_measure_peaks)."""
+        lds, hbm = C.c_double(), C.c_double()
+        _check(load().pgamd_measure_peaks(self.h, C.byref(lds), C.byref(hbm)))
+        return lds.value, hbm.value
 
     # --- reference call shape ---------------------------------------------------------------------------
     def mem_allocate(self, npatch_group: int):

@@ -90,6 +90,11 @@ int pgamd_set_options(pgamd_context* ctx, const pgamd_options* opt);
 const char* pgamd_last_error(void);
 /* Writes "<device name>;CUs=<n>;LDS/CU=<bytes>;clock_kHz=<n>" into buf. */
 int pgamd_device_info(pgamd_context* ctx, char* buf, int buflen);
+/* Measured peaks for the roofline report (no reference counterpart): LDS read bandwidth of the whole device
+ * (every CU streaming conflict-free 16-byte ds_read_b128 reads) and HBM copy bandwidth (1 GiB device copy,
+ * bytes read + written).  Either pointer may be NULL to skip that measurement.  Synchronous; allocates
+ * 2 GiB for the copy. */
+int pgamd_measure_peaks(pgamd_context* ctx, double* lds_read_bytes_per_s, double* hbm_copy_bytes_per_s);
 
 /* Replaces CUAPI_MemAllocate_PoissonGravity (GPU_API/CUAPI_MemAllocate_PoissonGravity.cu:27-80): device
  * buffers for 8*npatch_group patches at the context's ghost sizes, pinned host buffers handed to the
