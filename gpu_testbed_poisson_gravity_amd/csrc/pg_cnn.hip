@@ -17,11 +17,13 @@
 // Activations are stored padded (10^3 with a zero border) so that every layer is the same valid 3^3
 // convolution: 10^3 x 16 channels in, 8^3 x 16 channels out, K = 27 taps x 16 channels = 432.
 //
-// Convolutions are implicit GEMMs on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16): rows = output
-// voxels (16 per tile), columns = 16 output channels, K made of (tap, channel) pairs so that each lane's
-// 8 consecutive K values are 8 channels of one input voxel (one 16-byte LDS read).  MFMA lane layout
-// (16x16x32): A[i][k] with i = lane%16, k = 8*(lane/16)+0..7; B[k][j] with j = lane%16,
-// k = 8*(lane/16)+0..7; D[i][j] with j = lane%16, i = 4*(lane/16)+0..3.
+// Convolutions are implicit GEMMs on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16): the weights are
+// the A operand (rows = 16 output channels), the activations the B operand (columns = 16 output voxels of a
+// tile), K made of (tap, channel) pairs so that each lane's 8 consecutive K values are 8 channels of one input
+// voxel (one 16-byte LDS read).  MFMA lane layout (16x16x32): A[i][k] with i = lane%16, k = 8*(lane/16)+0..7;
+// B[k][j] with j = lane%16, k = 8*(lane/16)+0..7; D[i][j] with j = lane%16, i = 4*(lane/16)+0..3.  So a lane
+// loads the channels of voxel lane%16 and receives output channels 4*(lane/16)..+3 of that same voxel: the
+// epilogue reads and writes 4 consecutive channels (8 bytes) at once.
 // Forward K order: 15 steps c = dz*5 + s; step c covers the in-plane positions 2s and 2s+1 (position
 // dy*3 + dx, 16 channels each) of tap plane dz; position 9 is a pad with zero weights whose A operand
 // repeats position 8.  Grouping the steps by dz lets the fused kernel read an input plane once and feed it
@@ -49,6 +51,13 @@ typedef uint16_t u16;
 __device__ __forceinline__ float bf2f(u16 v) { return __uint_as_float(((uint32_t)v) << 16); }
 // round to nearest even (one v_cvt_pk_bf16_f32 on gfx950); every kernel here uses this one conversion
 __device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
+
+// Four floats as four BF16 values (8 bytes, the first one lowest).
+__device__ __forceinline__ uint2 pack4_bf16(float a, float b, float c, float d) {
+  return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
+}
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 
 union Pack8 {
   bf16x8 v;
@@ -115,12 +124,12 @@ __global__ void __launch_bounds__(kThreads) conv_fwd_kernel(const u16* __restric
   bf16x8 wreg[KZ];
   load_weights_fwd(W, wreg);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int co = lane & 15, g = lane >> 4;
-  const float b = bias[co];
+  const int g = lane >> 4, co0 = 4 * g;   // this lane's output channels co0..co0+3
+  const float4 b4 = *reinterpret_cast<const float4*>(bias + co0);
   __syncthreads();
   constexpr int M = kTZ * So * So;   // 128 output voxels: 8 tiles, 2 per wave
   for (int t = wave; t < M / 16; t += kThreads / 64) {
-    const int v = t * 16 + (lane & 15);
+    const int v = t * 16 + (lane & 15);   // this lane's voxel: its B-operand column and its D column
     const int lz = v / (So * So), oy = (v / So) % So, ox = v % So;
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -128,22 +137,24 @@ __global__ void __launch_bounds__(kThreads) conv_fwd_kernel(const u16* __restric
       const int dz = c / 5, pos = fwd_pos(c % 5, g), dy = pos / 3, dx = pos % 3;
       Pack8 a;
       a.u = *reinterpret_cast<const uint4*>(xs + (((lz + dz) * S + oy + dy) * S + ox + dx) * CH + 8 * (g & 1));
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, wreg[c], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[c], a.v, acc, 0, 0, 0);
     }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int vo = t * 16 + 4 * (lane >> 4) + r;
-      const int z = vo / (So * So), y = (vo / So) % So, x = vo % So;
-      const float h = acc[r] + b;
-      const size_t o = (((size_t)n * So + z0 + z) * So + y) * So + x;  // voxel in the 8^3 output
-      if (MODE == 2) {
-        if (co < kCnnOut) D[o * kCnnOut + co] = h;
-      } else {
-        float out = fmaxf(h, 0.0f);
-        if (MODE == 1) out += bf2f(xs[(((z + 1) * S + y + 1) * S + x + 1) * CH + co]);
-        Y[((((size_t)n * S + z0 + z + 1) * S + y + 1) * S + x + 1) * CH + co] = f2bf(out);
-        if (H) H[o * CH + co] = f2bf(h);
+    const float h0 = acc[0] + b4.x, h1 = acc[1] + b4.y, h2 = acc[2] + b4.z, h3 = acc[3] + b4.w;
+    const size_t o = (((size_t)n * So + z0 + lz) * So + oy) * So + ox;  // voxel in the 8^3 output
+    if (MODE == 2) {
+      if (co0 < kCnnOut) *reinterpret_cast<float4*>(D + o * kCnnOut + co0) = make_float4(h0, h1, h2, h3);
+    } else {
+      float o0 = fmaxf(h0, 0.0f), o1 = fmaxf(h1, 0.0f), o2 = fmaxf(h2, 0.0f), o3 = fmaxf(h3, 0.0f);
+      if (MODE == 1) {
+        const uint2 rx = *reinterpret_cast<const uint2*>(xs + (((lz + 1) * S + oy + 1) * S + ox + 1) * CH + co0);
+        o0 += bf_lo(rx.x);
+        o1 += bf_hi(rx.x);
+        o2 += bf_lo(rx.y);
+        o3 += bf_hi(rx.y);
       }
+      *reinterpret_cast<uint2*>(Y + ((((size_t)n * S + z0 + lz + 1) * S + oy + 1) * S + ox + 1) * CH + co0) =
+          pack4_bf16(o0, o1, o2, o3);
+      if (H) *reinterpret_cast<uint2*>(H + o * CH + co0) = pack4_bf16(h0, h1, h2, h3);
     }
   }
 }
@@ -567,10 +578,11 @@ struct FusedLds {
 };
 static_assert(sizeof(FusedLds) <= 160 * 1024 / 2, "two workgroups per CU");
 
+// Layer l's weights (A operand) and the biases of this lane's output channels 4*(lane/16)..+3.
 __device__ __forceinline__ void load_layer(const u16* __restrict__ Wb, const float* __restrict__ P, int l,
-                                           bf16x8 (&w)[KZ], float& b) {
+                                           bf16x8 (&w)[KZ], float4& b) {
   load_weights_fwd(Wb + (size_t)l * 16 * KF, w);
-  b = P[(size_t)l * kCnnLayerParams + 16 * K + (threadIdx.x & 15)];
+  b = *reinterpret_cast<const float4*>(P + (size_t)l * kCnnLayerParams + 16 * K + 4 * ((threadIdx.x & 63) >> 4));
 }
 
 // This lane's A-operand offsets (bf16 elements, relative to its row voxel in an input plane) for the five
@@ -584,27 +596,29 @@ __device__ __forceinline__ void pair_offsets(int (&off)[5]) {
   }
 }
 
-// Epilogue of output plane o for row pair rp: rows 4g..4g+3 of tile 4o + rp (4 consecutive voxels along x).
+// Epilogue of output plane o for row pair rp: output channels co0..co0+3 (co0 = 4*(lane/16)) of voxel
+// lane%16 of tile 4o + rp, 4 consecutive channels of one half-plane (8 bytes).
 template <int MODE>
-__device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, const floatx4& acc, float b, int o,
-                                           int rp) {
+__device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, const floatx4& acc, const float4& b,
+                                           int o, int rp) {
   const int lane = threadIdx.x & 63;
-  const int co = lane & 15, g = lane >> 4;
-  const int vo0 = (4 * o + rp) * 16 + 4 * g;
+  const int co0 = 4 * (lane >> 4);
+  const int vo = (4 * o + rp) * 16 + (lane & 15);
+  const float h0 = acc[0] + b.x, h1 = acc[1] + b.y, h2 = acc[2] + b.z, h3 = acc[3] + b.w;
   if (MODE == 2) {
-    if (co < kCnnOut) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) D[(vo0 + r) * kCnnOut + co] = acc[r] + b;
-    }
+    if (co0 < kCnnOut) *reinterpret_cast<float4*>(D + vo * kCnnOut + co0) = make_float4(h0, h1, h2, h3);
   } else {
-    const int q0 = (((vo0 >> 6) + 1) * S + ((vo0 >> 3) & 7) + 1) * S + (vo0 & 7) + 1;
-    const int c0 = (co >> 3) * kHalfP + q0 * 8 + (co & 7);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      float v = fmaxf(acc[r] + b, 0.0f);
-      if (MODE == 1) v += bf2f(in[c0 + 8 * r]);
-      out[c0 + 8 * r] = f2bf(v);
+    const int q = (((vo >> 6) + 1) * S + ((vo >> 3) & 7) + 1) * S + (vo & 7) + 1;
+    const int c0 = (co0 >> 3) * kHalfP + q * 8 + (co0 & 7);
+    float o0 = fmaxf(h0, 0.0f), o1 = fmaxf(h1, 0.0f), o2 = fmaxf(h2, 0.0f), o3 = fmaxf(h3, 0.0f);
+    if (MODE == 1) {
+      const uint2 rx = *reinterpret_cast<const uint2*>(in + c0);
+      o0 += bf_lo(rx.x);
+      o1 += bf_hi(rx.x);
+      o2 += bf_lo(rx.y);
+      o3 += bf_hi(rx.y);
     }
+    *reinterpret_cast<uint2*>(out + c0) = pack4_bf16(o0, o1, o2, o3);
   }
 }
 
@@ -615,7 +629,7 @@ __device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, co
 // (with 4 waves: 50 reads for 120 MFMAs per wave instead of one read per MFMA).  Output plane o still
 // accumulates its 15 steps in the order of conv_fwd_kernel: dz = 0, 1, 2, and s = 0..4 within each.
 template <int MODE>
-__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KZ], float b, float* D,
+__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KZ], const float4& b, float* D,
                                          const int (&off)[5]) {
   constexpr int NO = So / kZSplit, NQ = NO + 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15;
@@ -631,9 +645,9 @@ __device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (
     floatx4 a2 = zero;            // output plane P0+q
 #pragma unroll
     for (int s = 0; s < 5; s++) {
-      if (q < NO) a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[s], a2, 0, 0, 0);
-      if (q >= 1 && q <= NO) a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[5 + s], a1, 0, 0, 0);
-      if (q >= 2) a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s].v, wreg[10 + s], a0, 0, 0, 0);
+      if (q < NO) a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[s], a[s].v, a2, 0, 0, 0);
+      if (q >= 1 && q <= NO) a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[5 + s], a[s].v, a1, 0, 0, 0);
+      if (q >= 2) a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[10 + s], a[s].v, a0, 0, 0, 0);
     }
     if (q >= 2) conv_store<MODE>(in, out, D, a0, b, P0 + q - 2, rp);
     a0 = a1;
@@ -694,7 +708,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   u16* B = reinterpret_cast<u16*>(L.B);
   // first layer's weights, the coarse cube and this lane's share of the density are requested together
   bf16x8 wa[KZ], wb[KZ];
-  float ba, bb;
+  float4 ba, bb;
   load_layer(Wb, P, 0, wa, ba);
   for (int t = threadIdx.x; t < C3 / 4; t += kFT)
     reinterpret_cast<float4*>(L.c.cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
