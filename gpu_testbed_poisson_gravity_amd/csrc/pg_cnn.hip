@@ -46,15 +46,22 @@ namespace cnn {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 typedef uint16_t u16;
 
 __device__ __forceinline__ float bf2f(u16 v) { return __uint_as_float(((uint32_t)v) << 16); }
 // round to nearest even (one v_cvt_pk_bf16_f32 on gfx950); every kernel here uses this one conversion
 __device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
 
+// Two floats as two BF16 values in one word (the first one low): one two-source v_cvt_pk_bf16_f32, the same
+// rounding as f2bf.
+__device__ __forceinline__ uint32_t pack2_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((floatx2){a, b}, bf16x2));
+}
 // Four floats as four BF16 values (8 bytes, the first one lowest).
 __device__ __forceinline__ uint2 pack4_bf16(float a, float b, float c, float d) {
-  return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
+  return make_uint2(pack2_bf16(a, b), pack2_bf16(c, d));
 }
 __device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
@@ -603,7 +610,7 @@ __device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, co
                                            int o, int rp) {
   const int lane = threadIdx.x & 63;
   const int co0 = 4 * (lane >> 4);
-  const int vo = (4 * o + rp) * 16 + (lane & 15);
+  const int vo = (o * So + rp + 4 * ((lane & 15) >> 3)) * So + (lane & 7);
   const float h0 = acc[0] + b.x, h1 = acc[1] + b.y, h2 = acc[2] + b.z, h3 = acc[3] + b.w;
   if (MODE == 2) {
     if (co0 < kCnnOut) *reinterpret_cast<float4*>(D + vo * kCnnOut + co0) = make_float4(h0, h1, h2, h3);
@@ -634,7 +641,7 @@ __device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (
   constexpr int NO = So / kZSplit, NQ = NO + 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15;
   const int rp = wave & 3, P0 = (wave >> 2) * NO;
-  const u16* col = in + ((P0 * S + 2 * rp + (i >> 3)) * S + (i & 7)) * 8;   // plane P0, row voxel of lane i
+  const u16* col = in + ((P0 * S + rp + 4 * (i >> 3)) * S + (i & 7)) * 8;   // plane P0, row voxel of lane i
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   floatx4 a0 = zero, a1 = zero;   // output planes P0+q-2 and P0+q-1
 #pragma unroll
@@ -655,13 +662,26 @@ __device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (
   }
 }
 
-// Zero the border voxels (any coordinate 0 or 9) of both half-planes of a padded activation buffer.
+// Zero the border voxels (any coordinate 0 or 9) of both half-planes of a padded activation buffer.  The
+// 488 border voxels of a half-plane are enumerated directly: the planes z = 0 and z = 9 (100 each), then per
+// inner plane the rows y = 0 and y = 9 (10 each) and the x = 0 / x = 9 ends of the 8 inner rows.
 __device__ __forceinline__ void zero_border(uint4* buf) {
-  for (int i = threadIdx.x; i < kCnnPadVox * 2; i += kFT) {
-    const int h = i / kCnnPadVox, vox = i % kCnnPadVox;
-    const int x = vox % S, y = (vox / S) % S, z = vox / (S * S);
-    if (x == 0 || x == S - 1 || y == 0 || y == S - 1 || z == 0 || z == S - 1)
-      buf[h * (kHalfP / 8) + vox] = make_uint4(0, 0, 0, 0);
+  constexpr int SS = S * S, RING = 4 * S - 4, NB = 2 * SS + (S - 2) * RING;
+  static_assert(NB == kCnnPadVox - (S - 2) * (S - 2) * (S - 2), "border count");
+  for (int i = threadIdx.x; i < 2 * NB; i += kFT) {
+    const int h = i / NB;
+    int k = i % NB, vox;
+    if (k < 2 * SS) {
+      vox = k < SS ? k : k + (S - 2) * SS;
+    } else {
+      k -= 2 * SS;
+      const int z = 1 + k / RING, r = k % RING;
+      if (r < 2 * S)
+        vox = z * SS + (r < S ? r : (S - 1) * S + r - S);
+      else
+        vox = z * SS + (1 + (r - 2 * S) / 2) * S + ((r - 2 * S) & 1) * (S - 1);
+    }
+    buf[h * (kHalfP / 8) + vox] = make_uint4(0, 0, 0, 0);
   }
 }
 
@@ -712,13 +732,14 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   load_layer(Wb, P, 0, wa, ba);
   for (int t = threadIdx.x; t < C3 / 4; t += kFT)
     reinterpret_cast<float4*>(L.c.cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
-  constexpr int RV = (R3 / 4 + kFT - 1) / kFT;   // float4 of density per thread
-  float4 rv[RV];
+  // Thread t < R^2 owns the solve-interior column (x, y) = (t % R, t / R) for the packing and the guess; its
+  // density column is requested now (each wavefront load reads 64 consecutive floats).
+  constexpr int RR = R * R;
+  static_assert(RR <= kFT, "one thread per interior column");
+  const bool owns_col = threadIdx.x < RR;
+  float rcol[R];
 #pragma unroll
-  for (int k = 0; k < RV; k++) {
-    const int i4 = threadIdx.x + k * kFT;
-    rv[k] = i4 < R3 / 4 ? reinterpret_cast<const float4*>(rho)[i4] : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+  for (int z = 0; z < R; z++) rcol[z] = owns_col ? rho[z * RR + threadIdx.x] : 0.0f;
   __syncthreads();
   float mx = interp_cube_lds<GS>(L.c.cs, int_scheme, cube);
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
@@ -729,47 +750,69 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   for (int w = 1; w < kFT / 64; w++) mx = fmaxf(mx, L.red[w]);
   const float s = mx > 0.0f ? mx : 1.0f;
   const float inv = 1.0f / s;
-  // packed input: channel pair (q0, q1) of sub-cell d of voxel vox is one 32-bit word (q0 low) at
-  // half-plane d/4, in-half channels 2(d%4), 2(d%4)+1.  Solve-interior cells get q0 = interp/s and
-  // q1 = 64 r/s (r from the cube, still in B, and the density read above); the other frame cells get
-  // q0 = interp/s on the cube's Dirichlet layer, 0 outside the cube, and q1 = 0.  Both loops write disjoint
-  // words, so they share one barrier.
+  // Packed input over the 20^3 frame (frame cell e = cube cell + 1 + OFF): channel pair (q0, q1) of sub-cell
+  // d of voxel e/2 is one 32-bit word (q0 low) at half-plane d/4, in-half channels 2(d%4), 2(d%4)+1.
+  // Solve-interior cells get q0 = interp/s and q1 = 64 r/s (r from the cube, still in B, and the density);
+  // the other frame cells get q0 = interp/s on the cube's Dirichlet layer, 0 outside the cube, and q1 = 0.
+  // The column owners sweep z with the column's cube values in registers; the frame columns outside the
+  // interior are spread over all threads.  Both write disjoint words, so they share one barrier.
   uint32_t* A32 = reinterpret_cast<uint32_t*>(A);
+  constexpr int HW = kHalfP / 2;                 // 32-bit words per half-plane
+  constexpr int E = 2 * S, LO = 2 + OFF, HI = LO + R;   // frame edge, interior frame range [LO, HI)
+  if (owns_col) {
+    const int x = threadIdx.x % R, y = threadIdx.x / R;
+    const int ex = x + LO, ey = y + LO;
+    const int colw = (((ey >> 1) * S + (ex >> 1)) * 4) + (ey & 1) * 2 + (ex & 1);   // word of frame z = 0
+    const float* cc = cube + (y + 1) * N + (x + 1);                               // cube column, stride N^2
 #pragma unroll
-  for (int k = 0; k < RV; k++) {
-    const int i4 = threadIdx.x + k * kFT;
-    if (i4 < R3 / 4) {
-      const float e4[4] = {rv[k].x, rv[k].y, rv[k].z, rv[k].w};
+    for (int ez = 0; ez < LO; ez++) {   // below the interior: Dirichlet layer, then outside the cube
+      const int Z = ez - 1 - OFF;
+      const float c0 = Z >= 0 ? cc[Z * N * N] * inv : 0.0f;
+      A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = (uint32_t)f2bf(c0);
+    }
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int idx = 4 * i4 + e;
-        // interior cell -> extended frame index 2 + local + OFF -> coarse voxel (>>1), sub-cell (&1)
-        const int lx = idx % R, ly = (idx / R) % R, lz = idx / (R * R);
-        const int ex = lx + 2 + OFF, ey = ly + 2 + OFF, ez = lz + 2 + OFF;
-        const int vox = ((ez >> 1) * S + (ey >> 1)) * S + (ex >> 1);
-        const int d = ((ez & 1) * 2 + (ey & 1)) * 2 + (ex & 1);
-        const float c = cube[((lz + 1) * N + ly + 1) * N + lx + 1];
-        const float r = interp_residual<N>(cube, lx + 1, ly + 1, lz + 1, cst * e4[e]);
-        A32[(d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)] =
-            (uint32_t)f2bf(c * inv) | ((uint32_t)f2bf((r * inv) * kResScale) << 16);
-        L.c.in[idx] = c;
-      }
+    for (int ez = HI; ez < E; ez++) {   // above the interior
+      const int Z = ez - 1 - OFF;
+      const float c0 = Z < N ? cc[Z * N * N] * inv : 0.0f;
+      A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = (uint32_t)f2bf(c0);
+    }
+    float cm = cc[0], c = cc[N * N];
+#pragma unroll
+    for (int z = 0; z < R; z++) {
+      const float* cz = cc + (z + 1) * N * N;
+      const float cp = cz[N * N];
+      const float r = cp + cm + cz[N] + cz[-N] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
+      const int ez = z + LO;
+      A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] =
+          (uint32_t)f2bf(c * inv) | ((uint32_t)f2bf((r * inv) * kResScale) << 16);
+      L.c.in[(z * R + y) * R + x] = c;
+      cm = c;
+      c = cp;
     }
   }
-  for (int vox = threadIdx.x; vox < kCnnPadVox; vox += kFT) {
-    const int cx = vox % S, cy = (vox / S) % S, cz = vox / (S * S);
-#pragma unroll
-    for (int d = 0; d < 8; d++) {
-      const int X = 2 * cx - 1 + (d & 1) - OFF, Y = 2 * cy - 1 + ((d >> 1) & 1) - OFF, Z = 2 * cz - 1 + (d >> 2) - OFF;
-      const bool interior = X >= 1 && X < N - 1 && Y >= 1 && Y < N - 1 && Z >= 1 && Z < N - 1;
-      if (interior) continue;
-      float c0 = 0.0f;
-      if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) c0 = cube[(Z * N + Y) * N + X] * inv;
-      A32[(d >> 2) * (kHalfP / 2) + vox * 4 + (d & 3)] = (uint32_t)f2bf(c0);
+  constexpr int NOC = E * E - RR;   // frame columns outside the interior: rows below LO, above HI, then sides
+  for (int t = threadIdx.x; t < NOC * E; t += kFT) {
+    const int ez = t / NOC, j = t % NOC;
+    int ex, ey;
+    if (j < LO * E) {
+      ey = j / E;
+      ex = j % E;
+    } else if (j < E * E - HI * E + LO * E) {
+      ey = HI + (j - LO * E) / E;
+      ex = (j - LO * E) % E;
+    } else {
+      const int k = j - (E * E - HI * E + LO * E), w = E - R;
+      ey = LO + k / w;
+      ex = k % w < LO ? k % w : k % w - LO + HI;
     }
+    const int X = ex - 1 - OFF, Y = ey - 1 - OFF, Z = ez - 1 - OFF;
+    float c0 = 0.0f;
+    if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) c0 = cube[(Z * N + Y) * N + X] * inv;
+    A32[(ez & 1) * HW + (((ez >> 1) * S + (ey >> 1)) * S + (ex >> 1)) * 4 + (ey & 1) * 2 + (ex & 1)] =
+        (uint32_t)f2bf(c0);
   }
   __syncthreads();   // every lane is done reading the cube in B
-  for (int i = threadIdx.x; i < kBufVec; i += kFT) L.B[i] = make_uint4(0, 0, 0, 0);
+  zero_border(L.B);   // layer 1 writes every interior voxel of B
   int off[5];
   pair_offsets(off);
   __syncthreads();
@@ -792,12 +835,15 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   float* D = reinterpret_cast<float*>(L.A);
   conv_lds<2>(B, nullptr, wb, bb, D, off);     // layer 6: B -> D (in A)
   __syncthreads();
-  for (int t = threadIdx.x; t < R3; t += kFT) {
-    const int x = t % R, y = (t / R) % R, z = t / (R * R);
-    const int ix = x + OFF, iy = y + OFF, iz = z + OFF;
-    const int o = ((iz >> 1) * So + (iy >> 1)) * So + (ix >> 1);
-    const int p = ((iz & 1) * 2 + (iy & 1)) * 2 + (ix & 1);
-    guess[t] = L.c.in[t] + s * D[o * kCnnOut + p];
+  if (owns_col) {   // the guess, column by column (each wavefront store covers 64 consecutive floats)
+    const int x = threadIdx.x % R, y = threadIdx.x / R;
+    const int ix = x + OFF, iy = y + OFF;
+    const int colo = ((iy >> 1) * So + (ix >> 1)) * kCnnOut + (iy & 1) * 2 + (ix & 1);
+#pragma unroll
+    for (int z = 0; z < R; z++) {
+      const int iz = z + OFF, t = (z * R + y) * R + x;
+      guess[t] = L.c.in[t] + s * D[colo + (iz >> 1) * (So * So * kCnnOut) + (iz & 1) * 4];
+    }
   }
 }
 
