@@ -783,16 +783,17 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
       const float cp = cz[N * N];
       const float r = cp + cm + cz[N] + cz[-N] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
       const int ez = z + LO;
-      A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] =
-          (uint32_t)f2bf(c * inv) | ((uint32_t)f2bf((r * inv) * kResScale) << 16);
+      A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = pack2_bf16(c * inv, (r * inv) * kResScale);
       L.c.in[(z * R + y) * R + x] = c;
       cm = c;
       c = cp;
     }
   }
-  constexpr int NOC = E * E - RR;   // frame columns outside the interior: rows below LO, above HI, then sides
-  for (int t = threadIdx.x; t < NOC * E; t += kFT) {
-    const int ez = t / NOC, j = t % NOC;
+  // Frame columns outside the interior (rows below LO and above HI, then the sides of the interior rows),
+  // as half-columns of E/2 cells: item i is column i/2, cells ez in [(i&1)*E/2, (i&1)*E/2 + E/2).
+  constexpr int NOC = E * E - RR;
+  for (int i = threadIdx.x; i < 2 * NOC; i += kFT) {
+    const int j = i >> 1, z0 = (i & 1) * (E / 2);
     int ex, ey;
     if (j < LO * E) {
       ey = j / E;
@@ -805,14 +806,19 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
       ey = LO + k / w;
       ex = k % w < LO ? k % w : k % w - LO + HI;
     }
-    const int X = ex - 1 - OFF, Y = ey - 1 - OFF, Z = ez - 1 - OFF;
-    float c0 = 0.0f;
-    if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) c0 = cube[(Z * N + Y) * N + X] * inv;
-    A32[(ez & 1) * HW + (((ez >> 1) * S + (ey >> 1)) * S + (ex >> 1)) * 4 + (ey & 1) * 2 + (ex & 1)] =
-        (uint32_t)f2bf(c0);
+    const int X = ex - 1 - OFF, Y = ey - 1 - OFF;
+    const bool xy_in = X >= 0 && X < N && Y >= 0 && Y < N;
+    const float* cc = cube + (xy_in ? Y * N + X : 0);
+    uint32_t* w0 = A32 + ((ey >> 1) * S + (ex >> 1)) * 4 + (ey & 1) * 2 + (ex & 1);
+#pragma unroll
+    for (int dz = 0; dz < E / 2; dz++) {
+      const int ez = z0 + dz, Z = ez - 1 - OFF;
+      const float c0 = xy_in && Z >= 0 && Z < N ? cc[Z * N * N] * inv : 0.0f;
+      w0[(ez & 1) * HW + (ez >> 1) * (S * S * 4)] = (uint32_t)f2bf(c0);
+    }
   }
   __syncthreads();   // every lane is done reading the cube in B
-  zero_border(L.B);   // layer 1 writes every interior voxel of B
+  for (int i = threadIdx.x; i < kBufVec; i += kFT) L.B[i] = make_uint4(0, 0, 0, 0);
   int off[5];
   pair_offsets(off);
   __syncthreads();
