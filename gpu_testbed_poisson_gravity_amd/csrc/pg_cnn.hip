@@ -685,12 +685,15 @@ __device__ __forceinline__ void zero_border(uint4* buf) {
   }
 }
 
-// Interpolated solve cube from a coarse cube already staged in LDS (same arithmetic as interp_cube).
-// Returns this thread's max |value| over the cells it wrote (the cube's max once reduced over the workgroup).
+// Interpolated solve cube from a coarse cube already staged in LDS (same arithmetic as interp_cube), into a
+// cube stored with pitch N + 2 and one spare cell on each side (cube cell (X, Y, Z) at
+// cube[(Z * (N + 2) + Y) * (N + 2) + X], -1 <= X, Y, Z <= N), so that the children that fall outside the solve
+// cube land in the spare layer and every child is stored without a test.  Returns this thread's max |value|
+// over the children inside the solve cube (the cube's max once reduced over the workgroup).
 template <int GS>
 __device__ float interp_cube_lds(const float* cs, int int_scheme, float* cube) {
   constexpr Geo g = geo_of(GS);
-  constexpr int N = g.N, C = g.C, U = g.U, NC = C - 2;
+  constexpr int N = g.N, NP = N + 2, C = g.C, U = g.U, NC = C - 2;
   float mx = 0.0f;
   for (int q = threadIdx.x; q < NC * NC * NC; q += kFT) {
     const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
@@ -700,14 +703,17 @@ __device__ float interp_cube_lds(const float* cs, int int_scheme, float* cube) {
         for (int di = 0; di < 3; di++) v[dk][dj][di] = cs[((k + dk - 1) * C + (j + dj - 1)) * C + (i + di - 1)];
     float ch[2][2][2];
     interp_children(v, int_scheme, ch);
+    const int X0 = 2 * (i - 1) - U, Y0 = 2 * (j - 1) - U, Z0 = 2 * (k - 1) - U;
+    const bool vx[2] = {X0 >= 0, X0 + 1 < N}, vy[2] = {Y0 >= 0, Y0 + 1 < N}, vz[2] = {Z0 >= 0, Z0 + 1 < N};
+    float* base = cube + (Z0 * NP + Y0) * NP + X0;
+#pragma unroll
     for (int cz = 0; cz < 2; cz++)
+#pragma unroll
       for (int cy = 0; cy < 2; cy++)
+#pragma unroll
         for (int cx = 0; cx < 2; cx++) {
-          const int X = 2 * (i - 1) + cx - U, Y = 2 * (j - 1) + cy - U, Z = 2 * (k - 1) + cz - U;
-          if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) {
-            cube[(Z * N + Y) * N + X] = ch[cz][cy][cx];
-            mx = fmaxf(mx, fabsf(ch[cz][cy][cx]));
-          }
+          base[(cz * NP + cy) * NP + cx] = ch[cz][cy][cx];
+          mx = fmaxf(mx, vx[cx] && vy[cy] && vz[cz] ? fabsf(ch[cz][cy][cx]) : 0.0f);
         }
   }
   return mx;
@@ -723,7 +729,9 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   static_assert(N * N * N * 4 <= kBufVec * 16 && kCnnOutputs * 4 <= kBufVec * 16, "scratch must fit a buffer");
   static_assert(C3 <= kCoarseMax && R3 <= kInteriorMax && R3 % 4 == 0 && C3 % 4 == 0, "float4 staging");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float* cube = reinterpret_cast<float*>(L.B);
+  constexpr int NP = N + 2;   // cube pitch (one spare cell on each side)
+  static_assert(NP * NP * NP * 4 <= kBufVec * 16, "padded cube must fit a buffer");
+  float* cube = reinterpret_cast<float*>(L.B) + (NP * NP + NP + 1);   // cube cell (0, 0, 0)
   u16* A = reinterpret_cast<u16*>(L.A);
   u16* B = reinterpret_cast<u16*>(L.B);
   // first layer's weights, the coarse cube and this lane's share of the density are requested together
@@ -763,25 +771,25 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
     const int x = threadIdx.x % R, y = threadIdx.x / R;
     const int ex = x + LO, ey = y + LO;
     const int colw = (((ey >> 1) * S + (ex >> 1)) * 4) + (ey & 1) * 2 + (ex & 1);   // word of frame z = 0
-    const float* cc = cube + (y + 1) * N + (x + 1);                               // cube column, stride N^2
+    const float* cc = cube + (y + 1) * NP + (x + 1);                              // cube column, stride NP^2
 #pragma unroll
     for (int ez = 0; ez < LO; ez++) {   // below the interior: Dirichlet layer, then outside the cube
       const int Z = ez - 1 - OFF;
-      const float c0 = Z >= 0 ? cc[Z * N * N] * inv : 0.0f;
+      const float c0 = Z >= 0 ? cc[Z * NP * NP] * inv : 0.0f;
       A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = (uint32_t)f2bf(c0);
     }
 #pragma unroll
     for (int ez = HI; ez < E; ez++) {   // above the interior
       const int Z = ez - 1 - OFF;
-      const float c0 = Z < N ? cc[Z * N * N] * inv : 0.0f;
+      const float c0 = Z < N ? cc[Z * NP * NP] * inv : 0.0f;
       A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = (uint32_t)f2bf(c0);
     }
-    float cm = cc[0], c = cc[N * N];
+    float cm = cc[0], c = cc[NP * NP];
 #pragma unroll
     for (int z = 0; z < R; z++) {
-      const float* cz = cc + (z + 1) * N * N;
-      const float cp = cz[N * N];
-      const float r = cp + cm + cz[N] + cz[-N] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
+      const float* cz = cc + (z + 1) * NP * NP;
+      const float cp = cz[NP * NP];
+      const float r = cp + cm + cz[NP] + cz[-NP] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
       const int ez = z + LO;
       A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = pack2_bf16(c * inv, (r * inv) * kResScale);
       L.c.in[(z * R + y) * R + x] = c;
@@ -808,12 +816,12 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
     }
     const int X = ex - 1 - OFF, Y = ey - 1 - OFF;
     const bool xy_in = X >= 0 && X < N && Y >= 0 && Y < N;
-    const float* cc = cube + (xy_in ? Y * N + X : 0);
+    const float* cc = cube + (xy_in ? Y * NP + X : 0);
     uint32_t* w0 = A32 + ((ey >> 1) * S + (ex >> 1)) * 4 + (ey & 1) * 2 + (ex & 1);
 #pragma unroll
     for (int dz = 0; dz < E / 2; dz++) {
       const int ez = z0 + dz, Z = ez - 1 - OFF;
-      const float c0 = xy_in && Z >= 0 && Z < N ? cc[Z * N * N] * inv : 0.0f;
+      const float c0 = xy_in && Z >= 0 && Z < N ? cc[Z * NP * NP] * inv : 0.0f;
       w0[(ez & 1) * HW + (ez >> 1) * (S * S * 4)] = (uint32_t)f2bf(c0);
     }
   }
