@@ -360,8 +360,28 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
 
   // B. owned cells into registers
   // The lane's two x-neighbours (xx = 0, 1) are adjacent in memory: one 8-byte load reads both, so every
-  // load instruction covers whole cache lines (no line is requested twice).
+  // load instruction covers whole cache lines (no line is requested twice).  All density and guess loads of
+  // the lane are issued before any is used: one memory round trip for the phase.
   LaneCells<GS> L;
+  float2 r2[2][R], g2[2][R];
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int zi = 0; zi < R; zi++) r2[yy][zi] = g2[yy][zi] = make_float2(0.0f, 0.0f);
+  if (active) {
+#pragma unroll
+    for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+      for (int zi = 0; zi < R; zi++)
+        r2[yy][zi] = *reinterpret_cast<const float2*>(lp.rho + (zi * R + 2 * yb + yy) * R + 2 * m);
+    if (P.init_mode == 2) {
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int zi = 0; zi < R; zi++)
+          g2[yy][zi] = *reinterpret_cast<const float2*>(lp.guess + (zi * R + 2 * yb + yy) * R + 2 * m);
+    }
+  }
 #pragma unroll
   for (int yy = 0; yy < 2; yy++) {
 #pragma unroll
@@ -371,10 +391,6 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
     }
 #pragma unroll
     for (int zi = 0; zi < R; zi++) {
-      const int gi = (zi * R + 2 * yb + yy) * R + 2 * m;
-      const float2 r2 = active ? *reinterpret_cast<const float2*>(lp.rho + gi) : make_float2(0.0f, 0.0f);
-      float2 g2 = make_float2(0.0f, 0.0f);
-      if (P.init_mode == 2 && active) g2 = *reinterpret_cast<const float2*>(lp.guess + gi);
 #pragma unroll
       for (int xx = 0; xx < 2; xx++) {
         const int c = (xx + yy + zi + 1) & 1;
@@ -384,9 +400,9 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
         else if (P.init_mode == 1)
           v = 0.0f;
         else
-          v = xx ? g2.y : g2.x;
+          v = xx ? g2[yy][zi].y : g2[yy][zi].x;
         L.phi[yy][xx][zi] = v;
-        L.cr[yy][xx][zi] = active ? P.cst * (xx ? r2.y : r2.x) : 0.0f;
+        L.cr[yy][xx][zi] = active ? P.cst * (xx ? r2[yy][zi].y : r2[yy][zi].x) : 0.0f;
         if (P.init_mode != 0 && active) lds[lb + own_off<GS>(c, zi + 1, yy, xx)] = v;
       }
     }
