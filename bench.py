@@ -25,9 +25,12 @@ HBM_PEAK = 8.0e12          # B/s, MI355X published
 LDS_PEAK = 256 * 128 * 2.4e9  # B/s: 256 CUs x 128 B/clk x 2.4 GHz, used when the measurement is unavailable
 MFMA_BF16_PEAK = 2.5e15    # FLOP/s, dense BF16, MI355X published
 LDS_BYTES_PER_UPDATE = 36  # 7 phi reads + 1 rho read + 1 write, FP32
-# Warm-start network FLOPs per patch: six 3^3 convolutions on the 8^3 grid, 16 channels in, 16 out
-# (the output layer's 8 live channels run on a 16-wide MFMA tile): 6 x 512 x 16 x 432 x 2 on the matrix cores
-CNN_MFMA_FLOP = 6 * 512 * 16 * 432 * 2
+# Warm-start network FLOPs per patch that the model needs: five 3^3 convolutions 16 -> 16 channels and the
+# 16 -> 8 output layer on the 8^3 grid, K = 27 taps x 16 channels = 432:
+#   5 x 512 x 16 x 432 x 2 + 512 x 8 x 432 x 2 = 38,928,384.
+# The roofline counts these only; the matrix cores also multiply the zero-weight K pad (480 of 432) and the
+# output layer's 8 unused rows, reported beside it as the issued figure.
+CNN_USEFUL_FLOP = 5 * 512 * 16 * 432 * 2 + 512 * 8 * 432 * 2
 WEIGHTS = os.path.join(ROOT, "gpu_testbed_poisson_gravity_amd", "weights", "warm_start_gs5.npy")
 # HBM bytes per launch of each kernel on the default workload, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
 # (each scaled by what the counter reads for a 1 GiB device copy); lines "traffic <kernel> <leg> <bytes>"
@@ -72,8 +75,8 @@ def parse():
     ap.add_argument("--gs", type=int, default=5)
     ap.add_argument("--check-every", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=16384,
-                    help="patches timed on the CPU oracle (0: skip; default: the whole 16384-patch workload, "
-                         "about a second on 16 host cores)")
+                    help="patches timed on the host cores by the reference CPU solver and by the oracle (0: skip; "
+                         "default: the whole 16384-patch workload, about a second each on 16 host cores)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--pad", type=int, default=None, help="SOR split-row padding (default: library choice)")
     ap.add_argument("--ppw", type=int, default=None, help="patches per wavefront (default: library choice)")
@@ -153,57 +156,83 @@ def run_mixed(a):
     d_it = torch.zeros(n, dtype=torch.int32, device=dev)
     d_st = torch.zeros(n, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream(dev)
-    graph = ctx.graph_mixed(plan, d_rho, d_cpot, d_out, d_flu, args, guess=d_guess, omega_by_ghost=omega,
-                            iters=d_it, status=d_st)
-
-    def step(e=None):
-        d_flu.copy_(d_flu0)
-        if e is not None:
-            e[0].record(stream)
-        if net is not None:
-            net.warm_start_mixed(plan, d_rho, d_cpot, args, d_guess, stream=stream.cuda_stream)
-        if e is not None:
-            e[1].record(stream)
-        graph.launch(stream.cuda_stream)
-        if e is not None:
-            e[2].record(stream)
-
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize(dev)
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
-    for s_ in range(a.steps):
-        step(evs[s_])
-    torch.cuda.synchronize(dev)
-    pre = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-    sol = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
-    it, st = d_it.cpu().numpy(), d_st.cpu().numpy()
     R = 6 + 2 * gs.astype(np.float64)
-    sor_lds = float((it * R ** 3 * LDS_BYTES_PER_UPDATE)[sch == pg.SOR].sum())
-    ms = pre + sol
+
+    def leg(init_mode):
+        """One start (zero, interpolation or the network's guess): a captured graph of the per-ghost-size
+        launches, timed with events around the network and around the graph."""
+        use_net = init_mode == pg.INIT_GUESS
+        ctx.set_options(init_mode=init_mode, guess_fallback_tol=a.fallback_tol if use_net else 0.0)
+        graph = ctx.graph_mixed(plan, d_rho, d_cpot, d_out, d_flu, args, guess=d_guess if use_net else None,
+                                omega_by_ghost=omega, iters=d_it, status=d_st)
+
+        def step(e=None):
+            d_flu.copy_(d_flu0)
+            if e is not None:
+                e[0].record(stream)
+            if use_net:
+                net.warm_start_mixed(plan, d_rho, d_cpot, args, d_guess, stream=stream.cuda_stream)
+            if e is not None:
+                e[1].record(stream)
+            graph.launch(stream.cuda_stream)
+            if e is not None:
+                e[2].record(stream)
+
+        for _ in range(a.warmup):
+            step()
+        torch.cuda.synchronize(dev)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
+        for s_ in range(a.steps):
+            step(evs[s_])
+        torch.cuda.synchronize(dev)
+        graph.close()
+        pre = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+        sol = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+        it, st = d_it.cpu().numpy(), d_st.cpu().numpy()
+        r = {"value": n / ((pre + sol) * 1e-3), "ms_per_step": pre + sol, "network_ms": pre, "solve_ms": sol,
+             "sor": {"mean_iters": float(it[sch == pg.SOR].mean()), "max_iters": int(it[sch == pg.SOR].max())},
+             "mg": {"mean_vcycles": float(it[sch == pg.MG].mean()), "max_vcycles": int(it[sch == pg.MG].max())},
+             "converged": int(((st & pg.STATUS_CONVERGED) != 0).sum()),
+             "fallbacks": int(((st & pg.STATUS_FALLBACK) != 0).sum()),
+             "fallbacks_sor": int(((st & pg.STATUS_FALLBACK) != 0)[sch == pg.SOR].sum()),
+             "fallbacks_mg": int(((st & pg.STATUS_FALLBACK) != 0)[sch == pg.MG].sum()),
+             "sor_lds_bytes": float((it * R ** 3 * LDS_BYTES_PER_UPDATE)[sch == pg.SOR].sum())}
+        log(f"[bench] {a.workload} start {init_mode}: {r['value']:.4g} patch-solves/s, network {pre:.3f} ms, "
+            f"solve {sol:.3f} ms, SOR {r['sor']['mean_iters']:.1f} iterations, MG {r['mg']['mean_vcycles']:.2f} "
+            f"V-cycles, fallbacks {r['fallbacks']}")
+        return r
+
+    legs = {}
+    if net is not None:            # config 5: the network against both starts without it
+        legs["zero"] = leg(pg.INIT_ZERO)
+        legs["interp"] = leg(pg.INIT_INTERP)
+        legs["warm"] = leg(pg.INIT_GUESS)
+        head = legs["warm"]
+    else:
+        head = legs["interp"] = leg(pg.INIT_INTERP)
+    sol = head["solve_ms"]
     out = {
         "metric": f"patch-solves/s at rel. residual <=1e-5, mixed ghost sizes ({a.workload}), 1 MI355X",
-        "value": n / (ms * 1e-3), "unit": "patch-solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
+        "value": head["value"], "unit": "patch-solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{n} patches, GS uniform in 1..5, densities {'/'.join(kinds)}, half SOR (omega by "
                                f"GS {OMEGA_BY_GS}) half MG (2+2 smoothing), rel. residual 1e-5, fused gravity, one "
                                f"captured graph (one launch per ghost size and scheme), autotuned layouts {tuned}; start: "
                                + ("learned warm start (guess check 1e-2)" if net is not None else "interpolation"),
                    "npatch": n},
-        "network_ms": pre, "solve_ms": sol,
-        "sor": {"mean_iters": float(it[sch == pg.SOR].mean()), "max_iters": int(it[sch == pg.SOR].max())},
-        "mg": {"mean_vcycles": float(it[sch == pg.MG].mean()), "max_vcycles": int(it[sch == pg.MG].max())},
-        "converged": int(((st & pg.STATUS_CONVERGED) != 0).sum()),
-        "fallbacks": int(((st & pg.STATUS_FALLBACK) != 0).sum()),
-        "roofline": {"bound": "lds", "achieved": sor_lds / (sol * 1e-3) / 1e9, "peak": lds_peak / 1e9, "unit": "GB/s",
-                     "frac": sor_lds / (sol * 1e-3) / lds_peak, "traffic": None,
+        "network_ms": head["network_ms"], "solve_ms": sol, "sor": head["sor"], "mg": head["mg"],
+        "converged": head["converged"], "fallbacks": head["fallbacks"],
+        "legs": {k: {q: v for q, v in r.items() if q != "sor_lds_bytes"} for k, r in legs.items()},
+        "warm_vs_zero": legs["warm"]["value"] / legs["zero"]["value"] if "warm" in legs else None,
+        "warm_vs_interp": legs["warm"]["value"] / legs["interp"]["value"] if "warm" in legs else None,
+        "roofline": {"bound": "lds", "achieved": head["sor_lds_bytes"] / (sol * 1e-3) / 1e9, "peak": lds_peak / 1e9,
+                     "unit": "GB/s", "frac": head["sor_lds_bytes"] / (sol * 1e-3) / lds_peak, "traffic": None,
                      "peak_basis": "measured LDS read bandwidth (pgamd_measure_peaks)" if peaks else "placeholder",
                      "note": "SOR patches' algorithmic LDS bytes over the whole graph time (MG time included)"},
         "cpu_baseline": None, "peaks_measured": peaks,
     }
     print(json.dumps(out))
-    graph.close()
     plan.close()
     if net is not None:
         net.close()
@@ -312,10 +341,14 @@ def main():
                 "hbm_algorithmic_bytes_per_launch": hbm,
                 "hbm_achieved": hbm / (r["solve_ms"] * 1e-3) / 1e9, "hbm_peak": HBM_PEAK / 1e9}
 
-    plain = warm = warm_serial = None
+    plain = interp = warm = warm_serial = None
     if a.mode in ("both", "plain"):
         plain = run(pg.INIT_ZERO, None)
         log(f"[bench] plain: {plain['value']:.4g} patch-solves/s, {plain['mean_iters']:.1f} iterations")
+        # the reference's own start (the interpolated coarse potential, CPU_PoissonSolver_SOR.cpp:65): the
+        # method without the network
+        interp = run(pg.INIT_INTERP, None)
+        log(f"[bench] interp: {interp['value']:.4g} patch-solves/s, {interp['mean_iters']:.1f} iterations")
     net = None
     if a.mode in ("both", "warm"):
         if os.path.exists(a.weights):
@@ -342,32 +375,52 @@ def main():
         roofs["sor_plain"] = lds_roof(plain, "plain")
     roof = roof_sor
     if warm_serial is not None:
-        flop = CNN_MFMA_FLOP * a.npatch
+        flop = CNN_USEFUL_FLOP * a.npatch
         ach = flop / (warm_serial["network_ms"] * 1e-3)
         roofs["cnn"] = {"kernel": "fused_guess_kernel (interpolation, packing, 6 MFMA convolutions in LDS, guess)",
                         "bound": "mfma", "achieved": ach / 1e12, "peak": MFMA_BF16_PEAK / 1e12, "unit": "TFLOP/s",
                         "frac": ach / MFMA_BF16_PEAK, "traffic": traffic.get(("fused_guess_kernel", "warm")),
-                        "algorithmic_flop_per_launch": flop, "launch_ms": warm_serial["network_ms"],
+                        "algorithmic_flop_per_launch": flop, "flop_basis": "useful: 38.93 MFLOP per patch",
+                        "launch_ms": warm_serial["network_ms"],
                         "hbm_algorithmic_bytes_per_launch": a.npatch * 4.0 * (2 * R ** 3 + g["C"] ** 3)}
         if warm_serial["network_ms"] > warm_serial["solve_ms"]:
             roof = roofs["cnn"]
 
-    cpu = None
+    cpu = cpu_port = None
     if a.cpu_sample > 0:
         import oracle as O
 
         n = min(a.cpu_sample, a.npatch)
+        n -= n % 8                                              # whole patch groups for the reference entry
         nt = len(os.sched_getaffinity(0))
         nt = min(nt, int(os.environ.get("OMP_NUM_THREADS") or nt))   # the job's CPU share
+        sample = (f"all {n} patches of the workload" if n == a.npatch else f"first {n} of the {a.npatch} patches")
+        # the reference's own CPU solver (CPU_PoissonGravitySolver, compiled from its sources by oracle/Makefile
+        # at -O3, OpenMP over patches): interpolated start and the stall rule only, so it runs the plain leg's
+        # mean iteration count as a fixed count (Min_Iter = Max_Iter), then the gravity update
+        k = int(round(plain["mean_iters"])) if plain is not None else 52
+        if O.ref_available("SOR", a.gs, 1, "f32"):
+            ref = O.RefSolver("SOR", a.gs, 1, "f32")
+            t = time.perf_counter()
+            ref.solve(rho[:n], cpot[:n], flu[:n], n // 8, 1e-3, 1.0, k, k, 1.8, 1, 1, 1, 1e-6, poi, pg.INT_CQUAD,
+                      nthreads=nt)
+            dt = time.perf_counter() - t
+            cpu = {"value": n / dt, "unit": "patch-solves/s", "cores": nt, "kind": "reference",
+                   "sample": sample + f", the reference CPU_PoissonGravitySolver (g++ -O3, OpenMP over patches, "
+                                      f"{nt} threads), SOR omega 1.8 at {k} iterations (the plain leg's mean) from "
+                                      "its interpolated start, then the gravity update",
+                   "iterations": k, "seconds": dt}
         p = O.make_params(gs=a.gs, init_mode=O.INIT_ZERO, term_mode=O.TERM_RELRES, check_every=a.check_every,
                           rel_tol=1e-5, sor_min_iter=10, sor_max_iter=1000, sor_omega=1.8, poi_coeff=poi, dt=1e-3)
         t = time.perf_counter()
         _, _, it_cpu, _ = O.solve(rho[:n], cpot[:n], flu[:n], p, nthreads=nt)
         dt = time.perf_counter() - t
-        cpu = {"value": n / dt, "unit": "patch-solves/s", "cores": nt, "kind": "port",
-               "sample": (f"all {n} patches of the workload" if n == a.npatch else f"first {n} of the {a.npatch} patches")
-                          + ", zero guess, FP32 oracle (C, -O3), OpenMP over patches",
-               "mean_iters": float(it_cpu.mean())}
+        cpu_port = {"value": n / dt, "unit": "patch-solves/s", "cores": nt, "kind": "port",
+                    "sample": sample + ", zero guess to relative residual 1e-5, the FP32 oracle (C++ restatement, "
+                                       "g++ -O3, OpenMP over patches)",
+                    "mean_iters": float(it_cpu.mean()), "seconds": dt}
+        if cpu is None:
+            cpu = cpu_port
 
     def strip(r):
         return None if r is None else {k: v for k, v in r.items() if k != "iters"}
@@ -381,10 +434,12 @@ def main():
                                + ("learned warm start (network, then SOR from its guess with the guess check at 1e-2)"
                                   if warm is not None else "zero guess (no trained weights)"),
                    "npatch": a.npatch, "gs": a.gs, "check_every": a.check_every},
-        "plain": strip(plain), "warm": strip(warm),
+        "plain": strip(plain), "interp": strip(interp), "warm": strip(warm),
         "warm_vs_plain": (warm["value"] / plain["value"]) if (warm is not None and plain is not None) else None,
+        "warm_vs_interp": (warm["value"] / interp["value"]) if (warm is not None and interp is not None) else None,
         "wall_ms_per_step": head_r["wall_ms_per_step"],
-        "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu, "peaks_measured": peaks,
+        "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu, "cpu_port": cpu_port,
+        "peaks_measured": peaks,
     }
     print(json.dumps(out))
     if net is not None:

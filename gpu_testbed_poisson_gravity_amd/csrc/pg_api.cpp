@@ -134,6 +134,7 @@ struct pgamd_context {
   int* h_status = nullptr;
   int last_npatch = 0;
   size_t rho_n = 0, cpot_n = 0, out_n = 0;  // floats per patch at allocation time
+  int alloc_gs = 0, alloc_gg = 0;           // ghost sizes the device buffers were sized for
   hipStream_t aux = nullptr;                // forked launches (mixed batches)
   hipStream_t capture = nullptr;            // graph capture
   hipEvent_t ev[2] = {nullptr, nullptr};    // fork / join
@@ -154,6 +155,7 @@ void free_device(pgamd_context* c) {
   c->d_iters = c->d_status = nullptr;
   c->h_iters = c->h_status = nullptr;
   c->cap = 0;
+  c->alloc_gs = c->alloc_gg = 0;
 }
 
 int ensure_streams(pgamd_context* c, int n) {
@@ -261,6 +263,8 @@ int pgamd_mem_allocate_poisson_gravity(pgamd_context* c, int npatch_group, float
   c->rho_n = (size_t)g.R * g.R * g.R;
   c->cpot_n = (size_t)g.C * g.C * g.C;
   c->out_n = (size_t)G * G * G;
+  c->alloc_gs = c->opt.pot_ghost_size;
+  c->alloc_gg = c->opt.gra_ghost_size;
   const size_t flu_n = (size_t)kNComp * kPatchCells;
   if (hipMalloc(&c->d_rho, np * c->rho_n * 4) != hipSuccess || hipMalloc(&c->d_pot_in, np * c->cpot_n * 4) != hipSuccess ||
       hipMalloc(&c->d_pot_out, np * c->out_n * 4) != hipSuccess || hipMalloc(&c->d_flu, np * flu_n * 4) != hipSuccess ||
@@ -303,6 +307,17 @@ int pgamd_asyn_poisson_gravity_solver(pgamd_context* c, const float* h_rho, floa
                                       int int_scheme, int p5_gradient, int poisson, int graacc, int gpu_nstream) {
   if (!c) return fail(PGAMD_ERR_ARG, "null context");
   if (c->cap == 0) return fail(PGAMD_ERR_STATE, "pgamd_mem_allocate_poisson_gravity has not been called");
+  // The device buffers and the per-patch copy strides were sized for the ghost sizes current at allocation;
+  // the kernels index with the current ones.  The reference fixes both at compile time
+  // (CUAPI_MemAllocate_PoissonGravity.cu:27-80), so a change in between is refused, never run.
+  if (c->opt.pot_ghost_size != c->alloc_gs || c->opt.gra_ghost_size != c->alloc_gg)
+    return fail(PGAMD_ERR_STATE,
+                "ghost sizes changed since pgamd_mem_allocate_poisson_gravity (allocated for %d/%d, now %d/%d): "
+                "allocate again",
+                c->alloc_gs, c->alloc_gg, c->opt.pot_ghost_size, c->opt.gra_ghost_size);
+  // the reference call shape has no guess array
+  if (poisson && c->opt.init_mode == PGAMD_INIT_GUESS)
+    return fail(PGAMD_ERR_ARG, "init_mode GUESS needs a guess array: use pgamd_solve_device");
   if (gpu_nstream < 1) return fail(PGAMD_ERR_ARG, "GPU_NStream %d < 1", gpu_nstream);
   const int npatch = 8 * npatch_group;
   if (npatch_group < 0 || npatch > c->cap)

@@ -301,6 +301,45 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
     }
   __syncthreads();
 
+  // Reject a caller guess whose relative residual sum|R| / sum|phi| (the SOR residual in the reference
+  // operand order, Const = Poi_Coeff dh^2, over the interior) exceeds guess_tol: start from the zero guess,
+  // as the SOR kernel does, and report it in the status.
+  bool fell_back = false;
+  if (P.init_mode == 2 && P.guess_tol > 0.0f) {
+    float num = 0.0f, den = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int s = 0; s < M::PER_THREAD; s++) {
+        const int q = threadIdx.x + kMgThreads * s;
+        if (q < M::CELLS0) {
+          int x, y, z;
+          cell0<GS>(q, c, x, y, z);
+          const float* p = sol0 + nat(N, x, y, z);
+          const float r = p[N * N] + p[-N * N] + p[N] + p[-N] + p[1] + p[-1] - 6.0f * p[0] -
+                          P.cst * rho[((z - 1) * R + (y - 1)) * R + (x - 1)];
+          num += fabsf(r);
+          den += fabsf(p[0]);
+        }
+      }
+    block_sum2(num, den, red);
+    fell_back = !(num <= P.guess_tol * den);
+    if (fell_back) {
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+#pragma unroll
+        for (int s = 0; s < M::PER_THREAD; s++) {
+          const int q = threadIdx.x + kMgThreads * s;
+          if (q < M::CELLS0) {
+            int x, y, z;
+            cell0<GS>(q, c, x, y, z);
+            sol0[nat(N, x, y, z)] = 0.0f;
+          }
+        }
+    }
+    __syncthreads();
+  }
+
   float dh[4];
 #pragma unroll
   for (int l = 0; l < 4; l++) dh[l] = l == 0 ? P.dh : P.dh * float(N - 1) / float(l < NLV ? M::n(l) - 1 : 1);
@@ -350,6 +389,7 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
   if (threadIdx.x == 0) {
     int st = error <= P.mg_tol ? kStatusConverged : kStatusMaxIter;
     if (!(error == error) || error == INFINITY) st |= kStatusNonFinite;
+    if (fell_back) st |= kStatusFallback;
     if (iters) *iters = iter;
     if (status) *status = st;
   }

@@ -13,31 +13,43 @@ namespace {
 constexpr int kLdsThreads = 256;
 constexpr int kLdsVec = 2048;        // uint4 per workgroup buffer (32 KB)
 constexpr int kLdsUnroll = 8;        // independent reads in flight per thread
-constexpr int kLdsStride = 264;      // uint4 between the unrolled reads; 8 * 264 = 2112 = 64 (mod 2048), so
-                                     // the addresses move on every iteration and no read can be hoisted
+constexpr int kLdsStride = 128;      // uint4 between the eight reads of one step (2 KB): with a base below 1024,
+                                     // base + 7 * 128 stays inside the buffer, so every address is the base
+                                     // register plus a constant offset of the instruction
 
-// Each lane reads one 16-byte word per step; the 64 lanes of a wavefront read 64 consecutive words (one
-// contiguous 1 KB span, no bank conflicts).  The reads are issued as ds_read_b128 instructions whose results
-// nothing consumes (volatile asm: kept, and no VALU work competes with them), eight in flight per wait.
-// The buffer is the kernel's dynamic LDS, which starts at LDS offset 0 (no static LDS here), so a word's byte
-// address is 16 times its index.
+// Each lane reads one 16-byte word per read; the 64 lanes of a wavefront read 64 consecutive words (one
+// contiguous 1 KB span, no bank conflicts).  Eight ds_read_b128 are issued back to back and waited for inside
+// one asm statement (s_waitcnt lgkmcnt(0) before the statement ends), so every destination register holds its
+// data when the compiler next sees it; the outputs are early-clobber, so no destination shares a register with
+// the address.  The base moves by 512 words every step (modulo 1024), so no read can be hoisted, and the
+// statement is volatile, so none is dropped; one component of the first read is folded into a running XOR for
+// the sink.  The buffer is the kernel's dynamic LDS, which starts at LDS offset 0 (no static LDS here), so a
+// word's byte address is 16 times its index.
 __global__ void __launch_bounds__(kLdsThreads) lds_read_kernel(int iters, uint32_t* sink) {
   extern __shared__ uint4 buf[];
+  static_assert(kLdsUnroll == 8 && kLdsStride * 16 == 2048, "the asm statement issues eight reads 2 KB apart");
   for (int i = threadIdx.x; i < kLdsVec; i += kLdsThreads)
     buf[i] = make_uint4(i, i * 3u + 1u, i * 5u + 2u, i * 7u + 3u);
   __syncthreads();
-  uint4 r[kLdsUnroll];
-  for (int it = 0; it < iters; it++) {
-#pragma unroll
-    for (int k = 0; k < kLdsUnroll; k++) {
-      const uint32_t addr = (uint32_t)((((it * kLdsUnroll + k) * kLdsStride + (int)threadIdx.x) & (kLdsVec - 1)) * 16);
-      asm volatile("ds_read_b128 %0, %1" : "=v"(r[k]) : "v"(addr));
-    }
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  }
   uint32_t a = 0;
-#pragma unroll
-  for (int k = 0; k < kLdsUnroll; k++) a ^= r[k].x ^ r[k].y ^ r[k].z ^ r[k].w;
+  for (int it = 0; it < iters; it++) {
+    const uint32_t base = (uint32_t)(((it * kLdsUnroll * 64 + (int)threadIdx.x) & 1023) * 16);
+    uint4 r0, r1, r2, r3, r4, r5, r6, r7;
+    asm volatile(
+        "ds_read_b128 %0, %8\n\t"
+        "ds_read_b128 %1, %8 offset:2048\n\t"
+        "ds_read_b128 %2, %8 offset:4096\n\t"
+        "ds_read_b128 %3, %8 offset:6144\n\t"
+        "ds_read_b128 %4, %8 offset:8192\n\t"
+        "ds_read_b128 %5, %8 offset:10240\n\t"
+        "ds_read_b128 %6, %8 offset:12288\n\t"
+        "ds_read_b128 %7, %8 offset:14336\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6), "=&v"(r7)
+        : "v"(base)
+        : "memory");
+    a ^= r0.x;
+  }
   if (a == 0x9e3779b9u) sink[blockIdx.x] = a;   // practically never true
 }
 

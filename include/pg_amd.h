@@ -44,7 +44,10 @@ extern "C" {
 #define PGAMD_INIT_GUESS 2
 
 /* SOR stopping rule.  REFERENCE: stop once the L1 residual of an iteration grows (after Min_Iter),
- * CPU_PoissonSolver_SOR.cpp:273-283.  RELRES: stop when sum|R| <= rel_tol * sum|phi| over the interior,
+ * CPU_PoissonSolver_SOR.cpp:273-283.  The device sums |R| as a wavefront tree, the reference CPU
+ * sequentially: where two successive totals are equal to within rounding, the stall can be detected an
+ * iteration or a few earlier or later than on the reference CPU (results then differ at FP32 round-off;
+ * at equal iteration counts they are bitwise equal).  RELRES: stop when sum|R| <= rel_tol * sum|phi| over the interior,
  * tested after max(1,Min_Iter) iterations and then every check_every iterations.  MG always uses the
  * reference's relative error dh^2 sum|Def| / sum|Sol| <= MG_Tolerated_Error. */
 #define PGAMD_TERM_REFERENCE 0
@@ -69,8 +72,8 @@ typedef struct {
   int check_every;      /* >= 1                                                    */
   float rel_tol;        /* RELRES tolerance                                        */
   int fuse_gravity;     /* 1: gravity update inside the solve kernel, phi still in LDS */
-  float guess_fallback_tol; /* init GUESS, SOR: use the zero guess instead when the guess's relative
-                               residual exceeds this (0: never) */
+  float guess_fallback_tol; /* init GUESS, SOR and MG: use the zero guess instead when the guess's relative
+                               residual sum|R| / sum|phi| exceeds this (0: never) */
   int mixed_persistent; /* mixed batches (pgamd_solve_mixed_device, graphs): 0 = one launch per ghost size and
                            scheme, one task per workgroup; 1 = one persistent work-queue launch per scheme.
                            Same results; 0 is faster on gfx950 (the persistent kernels spill registers) */
@@ -108,7 +111,9 @@ int pgamd_mem_free_poisson_gravity(pgamd_context* ctx, float** h_rho, float** h_
 /* Replaces CUAPI_Asyn_PoissonGravitySolver (GPU_API/CUAPI_Asyn_PoissonGravitySolver.cu:85-275); same
  * argument order.  Host arrays in (pinned ones from pgamd_mem_allocate_poisson_gravity overlap best);
  * returns before the GPU finishes: h_pot_out and h_flu are valid after pgamd_synchronize.  With
- * poisson == 0 and graacc != 0, h_pot_out is an input (:192-200). */
+ * poisson == 0 and graacc != 0, h_pot_out is an input (:192-200).  PGAMD_ERR_STATE if the context's ghost
+ * sizes differ from those at allocation (the device buffers were sized for those); PGAMD_ERR_ARG for
+ * init_mode PGAMD_INIT_GUESS (this call shape has no guess array). */
 int pgamd_asyn_poisson_gravity_solver(pgamd_context* ctx, const float* h_rho, float* h_pot_in, float* h_pot_out,
                                       float* h_flu, int npatch_group, float dt, float dh, int sor_min_iter,
                                       int sor_max_iter, float sor_omega, int mg_max_iter, int mg_npre_smooth,

@@ -314,6 +314,40 @@ def test_guess_fallback_to_zero_guess():
         assert (chk[3][k] & ~pg.STATUS_FALLBACK) == ref[3][k]
 
 
+def test_mg_guess_fallback_to_zero_guess():
+    """The MG twin of the SOR guess check: with guess_fallback_tol > 0 a guess whose relative residual
+    (sum|R| / sum|phi|, the SOR residual) exceeds the tolerance is replaced by the zero guess before the first
+    V-cycle and status bit 8 is set; an accepted guess is used as given."""
+    _torch()
+    gs, n = 4, 8
+    rho, cpot, flu, _ = I.plummer(n, gs, seed=5)
+    args = pg.solver_args(dh=1.0, poi_coeff=POI, mg_max_iter=3, mg_npre=2, mg_npost=2, mg_tol=0.0)
+    interior = O.interp(cpot, gs, pg.INT_CQUAD, dtype=np.float32)[:, 1:-1, 1:-1, 1:-1]
+    bad = (interior * 1.5 + np.float32(1e-3)).astype(np.float32)
+    bad[::2] = interior[::2]                                    # even patches keep the interpolated guess
+    with pg.Context(0) as c:
+        c.set_options(pot_ghost_size=gs, pot_scheme=pg.MG, init_mode=pg.INIT_ZERO)
+        zero = _solve_dev(c, rho, cpot, flu, args)
+        c.set_options(init_mode=pg.INIT_GUESS, guess_fallback_tol=0.0)
+        plain = _solve_dev(c, rho, cpot, flu, args, guess=_dev(bad))
+        c.set_options(guess_fallback_tol=0.05)
+        print("[case] MG guess fallback", flush=True)
+        chk = _solve_dev(c, rho, cpot, flu, args, guess=_dev(bad))
+    for k in (0, 1):
+        cube = O.interp(cpot[k:k + 1], gs, pg.INT_CQUAD, dtype=np.float32)[0]
+        cube[1:-1, 1:-1, 1:-1] = bad[k]
+        rr = O.relres(cube, rho[k], gs, POI, 1.0)
+        assert (rr > 0.05) == (k == 1), rr
+    fell = (chk[3] & pg.STATUS_FALLBACK) != 0
+    assert np.array_equal(fell, np.arange(n) % 2 == 1)
+    assert not np.any(plain[3] & pg.STATUS_FALLBACK)
+    for k in range(n):
+        ref = zero if fell[k] else plain
+        assert np.array_equal(chk[0][k], ref[0][k]) and np.array_equal(chk[1][k], ref[1][k])
+        assert chk[2][k] == ref[2][k]
+        assert (chk[3][k] & ~pg.STATUS_FALLBACK) == ref[3][k]
+
+
 def test_autotune_keeps_results_and_roundtrips(tmp_path):
     torch = _torch()
     gs, n = 4, 512

@@ -193,19 +193,91 @@ def test_reference_call_shape_host_arrays_multi_stream():
     assert np.all(st & (pg.STATUS_CONVERGED | pg.STATUS_MAX_ITER))
 
 
+def _dev_batch(gs, n, gg=1):
+    """Zeroed, correctly sized device arrays (rho, coarse potential, output window, fluid with rho = 1): a call
+    that host validation should reject would still only run a harmless solve if that validation regressed."""
+    import torch
+
+    g = pg.geometry(gs, gg)
+    dev = torch.device("cuda", 0)
+    flu = torch.zeros((n, 5, 8, 8, 8), device=dev)
+    flu[:, 0] = 1.0
+    return (torch.zeros((n, g["R"], g["R"], g["R"]), device=dev), torch.zeros((n, g["C"], g["C"], g["C"]), device=dev),
+            torch.zeros((n, g["G"], g["G"], g["G"]), device=dev), flu)
+
+
 def test_error_paths():
     with pg.Context(0) as ctx:
         with pytest.raises(pg.PgError):
             ctx.set_options(pot_ghost_size=6)
         ctx.set_options(pot_ghost_size=2, gra_ghost_size=1)
+        rho, cpot, out, flu = _dev_batch(2, 8)
         with pytest.raises(pg.PgError):  # 5-point gradient needs GRA_GHOST_SIZE = 2
-            ctx.solve_device(1, 1, 1, 1, 8, pg.solver_args(p5=True))
+            ctx.solve_device(rho, cpot, out, flu, 8, pg.solver_args(p5=True))
         with pytest.raises(pg.PgError):  # bad interpolation scheme
-            ctx.solve_device(1, 1, 1, 1, 8, pg.solver_args(int_scheme=3))
+            ctx.solve_device(rho, cpot, out, flu, 8, pg.solver_args(int_scheme=3))
         with pytest.raises(pg.PgError):  # solve before allocation
             ctx.asyn_poisson_gravity_solver(None, None, None, None, 1, 0.1, 1.0, 1, 1, 1.5, 1, 1, 1, 1e-6, POI, 4,
                                             False, True, True)
-        ctx.solve_device(None, None, 1, None, 0, pg.solver_args())  # zero patches: no-op
+        ctx.solve_device(None, None, out, None, 0, pg.solver_args())  # zero patches: no-op
+
+
+def test_asyn_refuses_ghost_sizes_changed_after_allocation():
+    """The host-array entry's device buffers are sized at allocation; solving after pgamd_set_options changed
+    a ghost size is refused with PGAMD_ERR_STATE before any copy or launch."""
+    with pg.Context(0) as ctx:
+        ctx.set_options(pot_ghost_size=1, gra_ghost_size=1)
+        h_rho, h_cpot, h_out, h_flu = ctx.mem_allocate(1)      # 8 patches at ghost size 1
+        h_rho[:] = 0.0
+        h_cpot[:] = 0.0
+        h_flu[:] = 1.0
+        call = lambda: ctx.asyn_poisson_gravity_solver(h_rho, h_cpot, h_out, h_flu, 1, 1e-3, 1.0, 5, 5, 1.5, 1, 1,  # noqa
+                                                        1, 1e-6, POI, pg.INT_CQUAD, False, True, True)
+        ctx.set_options(pot_ghost_size=5)
+        with pytest.raises(pg.PgError) as e:
+            call()
+        assert e.value.code == pg.ERR_STATE
+        ctx.set_options(pot_ghost_size=2, gra_ghost_size=2)
+        with pytest.raises(pg.PgError) as e:
+            call()
+        assert e.value.code == pg.ERR_STATE
+        ctx.set_options(pot_ghost_size=1, gra_ghost_size=1)   # back to the allocated sizes: runs
+        print("[case] asyn solve at the allocated ghost size", flush=True)
+        call()
+        ctx.synchronize()
+        assert np.all(h_out == 0.0)                         # zero density and boundary give a zero potential
+        ctx.mem_free()
+
+
+def test_guess_start_without_a_guess_array_is_refused():
+    """init_mode GUESS with no guess array: the reference-shaped entry (which has no guess argument) and the
+    graph capture refuse it with PGAMD_ERR_ARG; the autotuner times a guess start as the zero start."""
+    import torch
+
+    gs, n = 3, 64
+    with pg.Context(0) as ctx:
+        ctx.set_options(pot_ghost_size=gs, gra_ghost_size=1, init_mode=pg.INIT_GUESS)
+        h_rho, h_cpot, h_out, h_flu = ctx.mem_allocate(1)
+        h_flu[:] = 1.0
+        with pytest.raises(pg.PgError) as e:
+            ctx.asyn_poisson_gravity_solver(h_rho, h_cpot, h_out, h_flu, 1, 1e-3, 1.0, 5, 5, 1.5, 1, 1, 1, 1e-6, POI,
+                                            pg.INT_CQUAD, False, True, True)
+        assert e.value.code == pg.ERR_ARG
+        ctx.mem_free()
+        rho, cpot, out, flu = _dev_batch(gs, n)
+        with pytest.raises(pg.PgError) as e:
+            ctx.graph_device(rho, cpot, out, flu, n, pg.solver_args())
+        assert e.value.code == pg.ERR_ARG
+        r, c, f, _ = I.plummer(n, gs, seed=4)
+        rho.copy_(torch.from_numpy(r))
+        cpot.copy_(torch.from_numpy(c))
+        print("[case] autotune with init GUESS", flush=True)
+        ppw, pad, ms = ctx.autotune(gs, rho, cpot, out, n, pg.solver_args(sor_min_iter=10, sor_max_iter=10,
+                                                                           poi_coeff=POI, graacc=False), repeats=1)
+        assert ms > 0 and ppw in (1, 2, 4) and 0 <= pad <= 3
+        zero = pg.solve_host(r, c, None, gs=gs, init_mode=pg.INIT_ZERO, sor_min_iter=10, sor_max_iter=10,
+                             poi_coeff=POI, graacc=False)[0]
+        assert np.array_equal(out.cpu().numpy(), zero)        # the candidates ran the zero start
 
 
 def test_nonfinite_density_flagged():
@@ -251,5 +323,6 @@ def test_full_cube_output_window(scheme):
     assert np.array_equal(full[:, face], cube[:, face])
     with pg.Context(0) as ctx:
         ctx.set_options(pot_ghost_size=gs, gra_ghost_size=3)
+        rho_d, cpot_d, out_d, flu_d = _dev_batch(gs, 8, gg=3)
         with pytest.raises(pg.PgError):  # the gravity stencil needs GRA_GHOST_SIZE <= 2
-            ctx.solve_device(1, 1, 1, 1, 8, pg.solver_args(graacc=True))
+            ctx.solve_device(rho_d, cpot_d, out_d, flu_d, 8, pg.solver_args(graacc=True))
