@@ -84,6 +84,10 @@ def parse():
     ap.add_argument("--fallback-tol", type=float, default=1e-2,
                     help="guess check: a network guess with relative residual above this restarts from zero")
     ap.add_argument("--mode", choices=["both", "plain", "warm"], default="both")
+    ap.add_argument("--guess", choices=["cube", "interior"], default="cube",
+                    help="warm leg: the network writes the whole solve cube and the solve reads its Dirichlet layer "
+                         "from it (cube, init GUESS_CUBE), or the interior only (init GUESS: the solve "
+                         "interpolates the Dirichlet layer itself)")
     ap.add_argument("--chunks", type=int, default=0,
                     help="also time the pipelined warm start (pgamd_cnn_solve_device) over this many slices; "
                          "0: serial only (measured faster on MI355X: the two kernels do not share CUs well)")
@@ -91,6 +95,9 @@ def parse():
                     help="config2 (default): the metric's workload; config3: 32768 mixed-GS lognormal patches, "
                          "half SOR half MG, one captured graph; config5: 65536 mixed-GS "
                          "patches (power law / Plummer / lognormal), warm start with guess check, SOR+MG, gravity")
+    ap.add_argument("--asyn-streams", default="1,2,4,8",
+                    help="stream counts of the host-array pipeline leg (pgamd_asyn_poisson_gravity_solver with pinned "
+                         "buffers, PCIe included; empty: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size reference for "
                          "rocprof FETCH_SIZE / WRITE_SIZE calibration)")
@@ -239,6 +246,70 @@ def run_mixed(a):
     ctx.close()
 
 
+def asyn_pipeline(a, pg, torch, ctx, rho, cpot, flu, args, kernel_ms):
+    """The reference call shape with host arrays (CUAPI_Asyn_PoissonGravitySolver.cu:143-265): pinned buffers from
+    pgamd_mem_allocate_poisson_gravity, the batch split over GPU_NStream streams, each stream copying its slice in,
+    solving it and copying it out.  Reported: PCIe-inclusive patch-solves/s per stream count (host wall clock
+    around the call and pgamd_synchronize, median of the timed calls), and beside them the three phases alone on
+    the same bytes: host-to-device copies, the solve (the plain leg's kernel time) and device-to-host copies."""
+    n = a.npatch - a.npatch % 8
+    ctx.set_options(pot_ghost_size=a.gs, gra_ghost_size=1, pot_scheme=pg.SOR, init_mode=pg.INIT_ZERO,
+                    term_mode=pg.TERM_RELRES, check_every=a.check_every, rel_tol=1e-5, fuse_gravity=1,
+                    guess_fallback_tol=0.0)
+    h_rho, h_cpot, h_out, h_flu = ctx.mem_allocate(n // 8)
+    h_rho[:] = rho[:n]
+    h_cpot[:] = cpot[:n]
+    legs = {}
+    for ns in [int(v) for v in a.asyn_streams.split(",") if v]:
+        times = []
+        for k in range(a.warmup + a.steps):
+            h_flu[:] = flu[:n]                                   # identical work every call (outside the clock)
+            t0 = time.perf_counter()
+            ctx.asyn_poisson_gravity_solver(h_rho, h_cpot, h_out, h_flu, n // 8, args.dt, args.dh, args.sor_min_iter,
+                                            args.sor_max_iter, args.sor_omega, args.mg_max_iter, args.mg_npre_smooth,
+                                            args.mg_npost_smooth, args.mg_tolerated_error, args.poi_coeff,
+                                            args.int_scheme, False, True, True, gpu_nstream=ns)
+            ctx.synchronize()
+            if k >= a.warmup:
+                times.append(time.perf_counter() - t0)
+        ms = float(np.median(times)) * 1e3
+        legs[str(ns)] = {"ms": ms, "patch_solves_per_s": n / (ms * 1e-3)}
+        log(f"[bench] host-array pipeline, {ns} stream(s): {ms:.2f} ms per call, {n / (ms * 1e-3):.4g} patch-solves/s")
+    it, _ = ctx.patch_stats(n)
+    ctx.mem_free()
+    # the phases alone on the same bytes (pinned torch buffers, events on one stream)
+    dev = torch.device("cuda", 0)
+    h_in = [torch.from_numpy(x[:n]).pin_memory() for x in (rho, cpot, flu)]
+    d_in = [torch.empty_like(x, device=dev) for x in h_in]
+    d_out = [torch.empty((n, 10, 10, 10), device=dev), torch.empty((n, 5, 8, 8, 8), device=dev)]
+    h_o = [torch.empty(x.shape, pin_memory=True) for x in d_out]
+
+    def timed(fn):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        fn()
+        torch.cuda.synchronize(dev)
+        ts = []
+        for _ in range(max(3, a.steps // 4)):
+            ev[0].record()
+            fn()
+            ev[1].record()
+            torch.cuda.synchronize(dev)
+            ts.append(ev[0].elapsed_time(ev[1]))
+        return float(np.median(ts))
+
+    h2d = timed(lambda: [d.copy_(h, non_blocking=True) for d, h in zip(d_in, h_in)])
+    d2h = timed(lambda: [h.copy_(d, non_blocking=True) for h, d in zip(h_o, d_out)])
+    h2d_bytes = float(sum(x.numel() * 4 for x in h_in))
+    d2h_bytes = float(sum(x.numel() * 4 for x in d_out))
+    best = min(legs.values(), key=lambda r: r["ms"])
+    return {"npatch": n, "streams": legs, "h2d_ms": h2d, "kernel_ms": kernel_ms, "d2h_ms": d2h,
+            "h2d_GBps": h2d_bytes / (h2d * 1e-3) / 1e9, "d2h_GBps": d2h_bytes / (d2h * 1e-3) / 1e9,
+            "h2d_bytes": h2d_bytes, "d2h_bytes": d2h_bytes, "serial_sum_ms": h2d + kernel_ms + d2h,
+            "best_ms": best["ms"], "overlap_saving": 1.0 - best["ms"] / (h2d + kernel_ms + d2h),
+            "mean_iters": float(it.mean()),
+            "note": "PCIe-inclusive rate of the host-array entry: never the headline value"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -263,7 +334,8 @@ def main():
     d_flu0 = torch.from_numpy(flu).to(dev)
     d_flu = d_flu0.clone()
     d_out = torch.empty((a.npatch, 10, 10, 10), device=dev)
-    d_guess = torch.empty((a.npatch, R, R, R), device=dev)
+    N = g["N"]
+    d_guess = torch.empty((a.npatch, N, N, N) if a.guess == "cube" else (a.npatch, R, R, R), device=dev)
     d_it = torch.zeros(a.npatch, dtype=torch.int32, device=dev)
     d_st = torch.zeros(a.npatch, dtype=torch.int32, device=dev)
     poi = 4.0 * np.pi * 1e-5
@@ -300,7 +372,9 @@ def main():
                 net.solve_device(d_rho, d_cpot, d_guess, d_out, d_flu, a.npatch, args, iters=d_it, status=d_st,
                                  nchunk=a.chunks, stream=sp)
             else:
-                if net is not None:
+                if net is not None and a.guess == "cube":
+                    net.warm_start_cube(d_rho, d_cpot, a.npatch, args, d_guess, stream=sp)
+                elif net is not None:
                     net.warm_start(d_rho, d_cpot, a.npatch, args, d_guess, stream=sp)
                 if e is not None:
                     e[1].record(stream)
@@ -332,7 +406,11 @@ def main():
     def lds_roof(r, leg):
         alg = float(r["iters"].astype(np.float64).sum()) * R ** 3 * LDS_BYTES_PER_UPDATE
         ach = alg / (r["solve_ms"] * 1e-3)
-        hbm = a.npatch * 4.0 * (R ** 3 * (2 if leg == "warm" else 1) + g["C"] ** 3 + 10 ** 3 + 4 * 512 + 5 * 512)
+        # density, the start (coarse cube; or the guess: interior plus the coarse cube, or the whole cube),
+        # the output window and the fluid (5 fields read, 4 written)
+        start = {"plain": g["C"] ** 3, "interp": g["C"] ** 3}.get(
+            leg, N ** 3 if a.guess == "cube" else R ** 3 + g["C"] ** 3)
+        hbm = a.npatch * 4.0 * (R ** 3 + start + 10 ** 3 + 4 * 512 + 5 * 512)
         return {"kernel": "sor_uniform_kernel", "bound": "lds", "achieved": ach / 1e9, "peak": lds_peak / 1e9,
                 "unit": "GB/s", "frac": ach / lds_peak, "traffic": traffic.get(("sor_uniform_kernel", leg)),
                 "peak_basis": "measured LDS read bandwidth (pgamd_measure_peaks)" if peaks else "placeholder",
@@ -354,12 +432,12 @@ def main():
         if os.path.exists(a.weights):
             net = pg.CNN(ctx)
             net.load_params(a.weights)
-            warm_serial = run(pg.INIT_GUESS, net)
+            warm_serial = run(pg.INIT_GUESS_CUBE if a.guess == "cube" else pg.INIT_GUESS, net)
             log(f"[bench] warm (serial): {warm_serial['value']:.4g} patch-solves/s, "
                 f"{warm_serial['mean_iters']:.1f} iterations, network {warm_serial['network_ms']:.3f} ms + "
                 f"solve {warm_serial['solve_ms']:.3f} ms")
             warm = warm_serial
-            if a.chunks > 0:
+            if a.chunks > 0 and a.guess == "interior":
                 warm_pipe = run(pg.INIT_GUESS, net, pipelined=True)
                 log(f"[bench] warm (pipelined, {a.chunks} slices): {warm_pipe['value']:.4g} patch-solves/s, "
                     f"{warm_pipe['mean_iters']:.1f} iterations")
@@ -382,7 +460,8 @@ def main():
                         "frac": ach / MFMA_BF16_PEAK, "traffic": traffic.get(("fused_guess_kernel", "warm")),
                         "algorithmic_flop_per_launch": flop, "flop_basis": "useful: 38.93 MFLOP per patch",
                         "launch_ms": warm_serial["network_ms"],
-                        "hbm_algorithmic_bytes_per_launch": a.npatch * 4.0 * (2 * R ** 3 + g["C"] ** 3)}
+                        "hbm_algorithmic_bytes_per_launch":
+                            a.npatch * 4.0 * (R ** 3 + g["C"] ** 3 + (N ** 3 if a.guess == "cube" else R ** 3))}
         if warm_serial["network_ms"] > warm_serial["solve_ms"]:
             roof = roofs["cnn"]
 
@@ -422,6 +501,10 @@ def main():
         if cpu is None:
             cpu = cpu_port
 
+    asyn = None
+    if a.asyn_streams and plain is not None:
+        asyn = asyn_pipeline(a, pg, torch, ctx, rho, cpot, flu, args, plain["solve_ms"])
+
     def strip(r):
         return None if r is None else {k: v for k, v in r.items() if k != "iters"}
 
@@ -439,7 +522,7 @@ def main():
         "warm_vs_interp": (warm["value"] / interp["value"]) if (warm is not None and interp is not None) else None,
         "wall_ms_per_step": head_r["wall_ms_per_step"],
         "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu, "cpu_port": cpu_port,
-        "peaks_measured": peaks,
+        "peaks_measured": peaks, "asyn_pipeline": asyn,
     }
     print(json.dumps(out))
     if net is not None:

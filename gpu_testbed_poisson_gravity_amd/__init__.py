@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libpgamd.so")
 
 SOR, MG = 1, 2
 INT_CQUAD, INT_QUAD = 4, 5
-INIT_INTERP, INIT_ZERO, INIT_GUESS = 0, 1, 2
+INIT_INTERP, INIT_ZERO, INIT_GUESS, INIT_GUESS_CUBE = 0, 1, 2, 3
 TERM_REFERENCE, TERM_RELRES = 0, 1
 STATUS_CONVERGED, STATUS_MAX_ITER, STATUS_NONFINITE, STATUS_FALLBACK = 1, 2, 4, 8
 ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_NOMEM = -1, -2, -3, -4
@@ -63,6 +63,25 @@ class SolverArgs(C.Structure):
     ]
 
 
+class SolverArgsF64(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("dh", C.c_double),
+        ("sor_min_iter", C.c_int),
+        ("sor_max_iter", C.c_int),
+        ("sor_omega", C.c_double),
+        ("mg_max_iter", C.c_int),
+        ("mg_npre_smooth", C.c_int),
+        ("mg_npost_smooth", C.c_int),
+        ("mg_tolerated_error", C.c_double),
+        ("poi_coeff", C.c_double),
+        ("int_scheme", C.c_int),
+        ("p5_gradient", C.c_int),
+        ("poisson", C.c_int),
+        ("graacc", C.c_int),
+    ]
+
+
 # Every entry point of include/pg_amd.h with its ctypes signature.
 _FP = C.POINTER(C.c_float)
 _FPP = C.POINTER(_FP)
@@ -88,6 +107,8 @@ SIGNATURES = {
     "pgamd_get_layout": (C.c_int, [_CTX, C.c_int, _IP, _IP]),
     "pgamd_solve_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                      C.POINTER(SolverArgs), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pgamd_solve_device_f64": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                         C.POINTER(SolverArgsF64), C.c_void_p, C.c_void_p, C.c_void_p]),
     "pgamd_autotune": (C.c_int, [_CTX, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(SolverArgs),
                                  C.c_int, _IP, _IP, _FP]),
     "pgamd_autotune_save": (C.c_int, [_CTX, C.c_char_p]),
@@ -122,6 +143,8 @@ SIGNATURES = {
                                        C.c_void_p]),
     "pgamd_cnn_warm_start": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(SolverArgs),
                                        C.c_void_p, C.c_void_p]),
+    "pgamd_cnn_warm_start_cube": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(SolverArgs),
+                                            C.c_void_p, C.c_void_p]),
     "pgamd_cnn_warm_start_mixed": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.POINTER(SolverArgs), C.c_void_p, C.c_void_p]),
     "pgamd_cnn_solve_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -280,6 +303,11 @@ class Context:
         _check(load().pgamd_solve_device(self.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out), _vp(flu), npatch,
                                          C.byref(args), _vp(iters), _vp(status), stream))
 
+    def solve_device_f64(self, rho, cpot, pot_out, flu, npatch, args: "SolverArgsF64", guess=None, iters=None,
+                         status=None, stream=None):
+        _check(load().pgamd_solve_device_f64(self.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out), _vp(flu), npatch,
+                                             C.byref(args), _vp(iters), _vp(status), stream))
+
     def autotune(self, gs, rho, cpot, pot_out, npatch, args: SolverArgs, repeats=3):
         """Time every layout of ghost size gs on this batch and keep the fastest: (ppw, pad, ms)."""
         p, d, ms = C.c_int(), C.c_int(), C.c_float()
@@ -363,6 +391,40 @@ def pack(arrays, offsets, total):
         f = np.asarray(a, np.float32).ravel()
         out[o:o + f.size] = f
     return out
+
+
+def solver_args_f64(dt=1e-2, dh=1.0, sor_min_iter=10, sor_max_iter=60, sor_omega=1.69, mg_max_iter=10, mg_npre=3,
+                    mg_npost=3, mg_tol=1e-6, poi_coeff=4.0 * np.pi * 1e-5, int_scheme=INT_CQUAD, p5=False,
+                    poisson=True, graacc=True) -> SolverArgsF64:
+    return SolverArgsF64(dt, dh, sor_min_iter, sor_max_iter, sor_omega, mg_max_iter, mg_npre, mg_npost, mg_tol,
+                         poi_coeff, int_scheme, int(p5), int(poisson), int(graacc))
+
+
+def solve_host_f64(rho, cpot, flu, *, gs, gg=1, scheme=SOR, init_mode=INIT_INTERP, term_mode=TERM_REFERENCE,
+                   check_every=1, rel_tol=1e-5, fuse_gravity=True, guess=None, pot_out=None, device=0, **argkw):
+    """FP64 twin of solve_host (pgamd_solve_device_f64): double inputs, double outputs."""
+    import torch
+
+    args = solver_args_f64(**argkw)
+    g = geometry(gs, gg)
+    npatch = (rho if rho is not None else pot_out).shape[0]
+    with Context(device) as ctx:
+        ctx.set_options(pot_ghost_size=gs, gra_ghost_size=gg, pot_scheme=scheme, init_mode=init_mode,
+                        term_mode=term_mode, check_every=check_every, rel_tol=rel_tol, fuse_gravity=int(fuse_gravity))
+        dev = torch.device("cuda", device)
+        t = lambda a: None if a is None else torch.as_tensor(np.ascontiguousarray(a, np.float64)).to(dev)  # noqa
+        d_rho, d_cpot, d_guess = t(rho), t(cpot), t(guess)
+        d_flu = t(flu) if flu is not None else None
+        d_out = (t(pot_out) if pot_out is not None else
+                 torch.zeros((npatch, g["G"], g["G"], g["G"]), dtype=torch.float64, device=dev))
+        d_it = torch.zeros(npatch, dtype=torch.int32, device=dev)
+        d_st = torch.zeros(npatch, dtype=torch.int32, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        ctx.solve_device_f64(d_rho, d_cpot, d_out, d_flu, npatch, args, guess=d_guess, iters=d_it, status=d_st,
+                             stream=stream)
+        torch.cuda.synchronize(dev)
+        return (d_out.cpu().numpy(), None if d_flu is None else d_flu.cpu().numpy(), d_it.cpu().numpy(),
+                d_st.cpu().numpy())
 
 
 def solve_host(rho, cpot, flu, *, gs, gg=1, scheme=SOR, init_mode=INIT_INTERP, term_mode=TERM_REFERENCE,
@@ -452,6 +514,11 @@ class CNN:
     def warm_start(self, rho, cpot, npatch, args: SolverArgs, guess, stream=None):
         _check(load().pgamd_cnn_warm_start(self.ctx.h, self.h, _vp(rho), _vp(cpot), npatch, C.byref(args),
                                            _vp(guess), stream))
+
+    def warm_start_cube(self, rho, cpot, npatch, args: SolverArgs, guess_cube, stream=None):
+        """The guess as the whole solve cube [n][N^3] (for init_mode INIT_GUESS_CUBE)."""
+        _check(load().pgamd_cnn_warm_start_cube(self.ctx.h, self.h, _vp(rho), _vp(cpot), npatch, C.byref(args),
+                                                _vp(guess_cube), stream))
 
     def warm_start_mixed(self, plan, rho, cpot, args: SolverArgs, guess, stream=None):
         _check(load().pgamd_cnn_warm_start_mixed(self.ctx.h, self.h, plan.h, _vp(rho), _vp(cpot), C.byref(args),

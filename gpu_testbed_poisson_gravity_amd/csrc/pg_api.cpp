@@ -63,7 +63,7 @@ int validate_options(const pgamd_options& o) {
                 o.pot_ghost_size);
   if (o.pot_scheme != PGAMD_SOR && o.pot_scheme != PGAMD_MG)
     return fail(PGAMD_ERR_ARG, "pot_scheme %d is neither SOR (1) nor MG (2)", o.pot_scheme);
-  if (o.init_mode < 0 || o.init_mode > 2) return fail(PGAMD_ERR_ARG, "init_mode %d", o.init_mode);
+  if (o.init_mode < 0 || o.init_mode > 3) return fail(PGAMD_ERR_ARG, "init_mode %d", o.init_mode);
   if (o.term_mode < 0 || o.term_mode > 1) return fail(PGAMD_ERR_ARG, "term_mode %d", o.term_mode);
   if (o.check_every < 1) return fail(PGAMD_ERR_ARG, "check_every %d < 1", o.check_every);
   if (!(o.rel_tol >= 0.0f)) return fail(PGAMD_ERR_ARG, "rel_tol must be >= 0");
@@ -316,7 +316,7 @@ int pgamd_asyn_poisson_gravity_solver(pgamd_context* c, const float* h_rho, floa
                 "allocate again",
                 c->alloc_gs, c->alloc_gg, c->opt.pot_ghost_size, c->opt.gra_ghost_size);
   // the reference call shape has no guess array
-  if (poisson && c->opt.init_mode == PGAMD_INIT_GUESS)
+  if (poisson && c->opt.init_mode >= PGAMD_INIT_GUESS)
     return fail(PGAMD_ERR_ARG, "init_mode GUESS needs a guess array: use pgamd_solve_device");
   if (gpu_nstream < 1) return fail(PGAMD_ERR_ARG, "GPU_NStream %d < 1", gpu_nstream);
   const int npatch = 8 * npatch_group;
@@ -427,13 +427,67 @@ int pgamd_solve_device(pgamd_context* c, const float* d_rho, const float* d_pot_
   if (npatch == 0) return PGAMD_OK;
   if ((a->poisson && (!d_rho || !d_pot_in)) || !d_pot_out || (a->graacc && !d_flu))
     return fail(PGAMD_ERR_ARG, "null device array");
-  if (a->poisson && c->opt.init_mode == PGAMD_INIT_GUESS && !d_guess)
-    return fail(PGAMD_ERR_ARG, "init_mode GUESS needs a guess array");
+  if (a->poisson && c->opt.init_mode >= PGAMD_INIT_GUESS && !d_guess)
+    return fail(PGAMD_ERR_ARG, "init_mode GUESS / GUESS_CUBE needs a guess array");
   HIP_TRY(hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
   const SolveParams p = make_params(c->opt, *a);
   return pgamd::enqueue_solve(c, d_rho, d_pot_in, d_guess, d_pot_out, a->graacc ? d_flu : nullptr, npatch, *a, p,
                               d_iters, d_status, st);
+}
+
+int pgamd_solve_device_f64(pgamd_context* c, const double* d_rho, const double* d_pot_in, const double* d_guess,
+                           double* d_pot_out, double* d_flu, int npatch, const pgamd_solver_args_f64* a64,
+                           int* d_iters, int* d_status, void* stream) {
+  if (!c || !a64) return fail(PGAMD_ERR_ARG, "null argument");
+  // the checks of the FP32 entry, on the same settings
+  const pgamd_solver_args a{(float)a64->dt, (float)a64->dh, a64->sor_min_iter, a64->sor_max_iter,
+                            (float)a64->sor_omega, a64->mg_max_iter, a64->mg_npre_smooth, a64->mg_npost_smooth,
+                            (float)a64->mg_tolerated_error, (float)a64->poi_coeff, a64->int_scheme, a64->p5_gradient,
+                            a64->poisson, a64->graacc};
+  int rc = validate_args(c->opt, a, npatch);
+  if (rc != PGAMD_OK) return rc;
+  if (npatch == 0) return PGAMD_OK;
+  if ((a.poisson && (!d_rho || !d_pot_in)) || !d_pot_out || (a.graacc && !d_flu))
+    return fail(PGAMD_ERR_ARG, "null device array");
+  if (a.poisson && c->opt.init_mode >= PGAMD_INIT_GUESS && !d_guess)
+    return fail(PGAMD_ERR_ARG, "init_mode GUESS / GUESS_CUBE needs a guess array");
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  const pgamd_options& o = c->opt;
+  SolveParams64 p{};
+  p.cst = a64->poi_coeff * a64->dh * a64->dh;         // CPU_PoissonSolver_SOR.cpp:51 (real = double)
+  p.w6 = a64->sor_omega / 6.0;                        // CPU_PoissonSolver_SOR.cpp:52
+  p.damp = fabs(1.0 - a64->sor_omega);
+  p.rel_tol = o.rel_tol;
+  p.min_iter = a64->sor_min_iter;
+  p.max_iter = a64->sor_max_iter;
+  p.term_mode = o.term_mode;
+  p.check_every = o.check_every;
+  p.init_mode = o.init_mode;
+  p.int_scheme = a64->int_scheme;
+  p.gra_ghost = o.gra_ghost_size;
+  p.poi_coeff = a64->poi_coeff;
+  p.dh = a64->dh;
+  p.mg_tol = a64->mg_tolerated_error;
+  p.mg_max_iter = a64->mg_max_iter;
+  p.mg_npre = a64->mg_npre_smooth;
+  p.mg_npost = a64->mg_npost_smooth;
+  p.fuse_gravity = o.fuse_gravity && a64->graacc;
+  p.p5 = a64->p5_gradient;
+  p.gra_const = a64->p5_gradient ? a64->dt / (12.0 * a64->dh) : a64->dt / (2.0 * a64->dh);   // CPU_GravitySolver.cpp:23
+  p.guess_tol = o.guess_fallback_tol;
+  if (a.poisson) {
+    const UniformBatch64 b{d_rho,   d_pot_in, d_guess, d_pot_out, p.fuse_gravity ? d_flu : nullptr, d_iters, d_status,
+                           npatch,  o.pot_ghost_size, o.pot_scheme};
+    hipError_t e = launch_uniform64(b, p, st);
+    if (e != hipSuccess) return hip_fail(e, "FP64 solve kernel launch");
+  }
+  if (a.graacc && !(a.poisson && p.fuse_gravity)) {
+    hipError_t e = launch_gravity64(d_pot_out, d_flu, npatch, o.gra_ghost_size, p.gra_const, p.p5, st);
+    if (e != hipSuccess) return hip_fail(e, "FP64 gravity kernel launch");
+  }
+  return PGAMD_OK;
 }
 
 }  // extern "C"

@@ -45,9 +45,10 @@ hipError_t cnn_adam(float* p, const float* g, float* m, float* v, int n, float l
 hipError_t cnn_weight_copies(const float* P, uint16_t* Wb, uint16_t* WT, hipStream_t s);
 hipError_t cnn_prep(const float* rho, const float* cpot, int n, int gs, float cst, int int_scheme, uint16_t* x,
                     float* scale, hipStream_t s);
-// Interpolation, packing, the six layers and the guess in one launch (activations in LDS).
+// Interpolation, packing, the six layers and the guess in one launch (activations in LDS).  cube = 0: guess of
+// the interior [n][R^3]; cube = 1: the whole solve cube [n][N^3], the Dirichlet layer being the interpolation.
 hipError_t cnn_fused_guess(const float* rho, const float* cpot, const uint16_t* Wb, const float* P, int n, int gs,
-                           float cst, int int_scheme, float* guess, hipStream_t s);
+                           float cst, int int_scheme, float* guess, int cube, hipStream_t s);
 struct PatchDesc;
 hipError_t cnn_fused_guess_mixed(const float* rho, const float* cpot, const PatchDesc* desc, const uint16_t* Wb,
                                  const float* P, int n, float cst, int int_scheme, float* guess, hipStream_t s);

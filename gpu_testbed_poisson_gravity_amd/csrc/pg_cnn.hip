@@ -4,12 +4,13 @@
 //
 // Network (activations BF16 channel-last, weights BF16, accumulation and master weights FP32):
 //   frame   the 18^3 solve cube of ghost size 5 (smaller cubes centred, zeros around), with the fields
-//           q0 = interp/s and q1 = 64 r/s (solve interior only), s = max|interp|, r the SOR residual of
-//           the interpolated cube (reference operand order), i.e. the right-hand side of A e = r that the
-//           correction e to the interpolated start satisfies
+//           q0 = interp/s and q1 = 64 r/s on the solve interior and zero elsewhere, s = max|interp| over the
+//           solve cube, r the SOR residual of the interpolated cube (reference operand order), i.e. the
+//           right-hand side of A e = r that the correction e to the interpolated start satisfies (e has zero
+//           Dirichlet values, so the boundary enters through r alone)
 //   x0      [n][10^3][16]: space-to-depth of the zero-extended 20^3 frame (fine index f = 2c - 1 + d per
 //           axis, channel ((dz*2+dy)*2+dx)*2 + q), so that coarse voxels 1..8 hold the 2^3 blocks of the
-//           16^3 interior and voxels 0 and 9 hold the Dirichlet layer
+//           16^3 interior and the voxels with a coordinate 0 or 9 are zero (the first layer's padding)
 //   y1      relu(conv3(x0) + b1)                        10^3 -> 8^3, 16 channels
 //   yl      y_{l-1} + relu(conv3(pad(y_{l-1})) + bl)    l = 2..5, 8^3 (residual blocks, zero padding)
 //   D       conv3(pad(y5)) + b6                         8^3, 8 channels = the 2^3 sub-cells (linear)
@@ -458,7 +459,8 @@ __device__ void interp_cube(const float* __restrict__ cpot, int int_scheme, floa
   __syncthreads();
 }
 
-// One workgroup per patch: interpolate, take s = max|interp|, write the packed input.
+// One workgroup per patch: interpolate, take s = max|interp|, write the packed input (zero outside the solve
+// interior, so every voxel with a coordinate 0 or 9 is zero).
 template <int GS>
 __device__ void prep_patch(const float* __restrict__ rho, const float* __restrict__ cpot, float cst, int int_scheme,
                            u16* __restrict__ x, float* __restrict__ scale, float* cube) {
@@ -485,10 +487,9 @@ __device__ void prep_patch(const float* __restrict__ rho, const float* __restric
     // frame index f = 2c - 1 + d, cube index f - OFF
     const int X = 2 * cx - 1 + dx - OFF, Y = 2 * cy - 1 + dy - OFF, Z = 2 * cz - 1 + dz - OFF;
     float c0 = 0.0f, c1 = 0.0f;
-    if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) {
+    if (X >= 1 && X <= R && Y >= 1 && Y <= R && Z >= 1 && Z <= R) {   // the solve interior only
       c0 = cube[(Z * N + Y) * N + X] * inv;
-      if (X >= 1 && X <= R && Y >= 1 && Y <= R && Z >= 1 && Z <= R)
-        c1 = (interp_residual<N>(cube, X, Y, Z, cst * rho[((Z - 1) * R + (Y - 1)) * R + (X - 1)]) * inv) * kResScale;
+      c1 = (interp_residual<N>(cube, X, Y, Z, cst * rho[((Z - 1) * R + (Y - 1)) * R + (X - 1)]) * inv) * kResScale;
     }
     reinterpret_cast<uint32_t*>(x)[t] = (uint32_t)f2bf(c0) | ((uint32_t)f2bf(c1) << 16);
   }
@@ -550,40 +551,47 @@ __global__ void __launch_bounds__(kThreads) guess_kernel(const float* __restrict
 }
 
 // ---- fused inference: one workgroup per patch, activations resident in LDS ------------------------------
-// interp + packing -> six convolutions ping-ponging between two padded 10^3 x 16 BF16 buffers -> guess.
-// Each output value goes through exactly the MFMA sequence of conv_fwd_kernel, so the guess is bitwise
-// identical to prep + forward + guess with the layer-by-layer kernels.
+// interpolation -> packing -> six convolutions ping-ponging between two LDS buffers -> guess written by the
+// output layer's epilogue.  Each output value goes through the MFMA sequence of
+// conv_fwd_kernel with the products of the all-zero input planes (padded z = 0 and 9) left out (adding those
+// zero products to an accumulator that started at +0 leaves it bitwise unchanged), so the guess is bitwise
+// that of prep + forward + guess with the layer-by-layer kernels.
 //
-// * An activation buffer is split into two half-planes of 8 channels, [half][voxel][8] (16 KB each), so
-//   that the 16-byte A-operand reads of 8 consecutive lanes (8 consecutive voxels along x) are one
-//   contiguous 128-byte span.  The two rows of a tile start 160 bytes apart, so their spans share 32
-//   bytes of banks: measured about 3.6 extra LDS cycles per read; a pitch-8 layout without the x border
-//   removes that overlap but needs a select per read and measured slower.
-// * Global traffic is issued early and coalesced: the coarse cube is staged once (float4), the density is
-//   read with float4 loads and scattered into the packed input, and the interpolated interior is kept in
-//   LDS (over the coarse staging, which is dead by then) for the final guess.
+// * An activation buffer holds only the 8 planes of the output grid, each [half][10 x 10 voxels][8 channels]
+//   BF16 with a zero ring (y or x = 0 or 9), 25.6 KB.  The 16-byte A-operand reads of 8 lanes (8 consecutive
+//   voxels along x) are one 128-byte span; a tile pairs rows r and r + 4, 640 bytes apart (128 modulo the
+//   256-byte bank width), so the 16 lanes of a read phase cover all 64 banks once.
+// * The packed input (the solve interior only; its border voxels are zero like the activations' padding)
+//   has the same layout, so the first layer is a layer like the others.
+// * Wave w owns output rows w and w + 4 of every plane and sweeps the 8 input planes once; every A operand
+//   read from LDS feeds up to three MFMAs, and the next plane's operands are requested before the current
+//   plane's MFMAs.
+// * The output layer's epilogue adds s * D to the interpolated interior (kept in LDS) and stores the guess.
+// LDS per workgroup: three buffers (76.8 KB), two workgroups per CU.  Measured on MI355X against computing
+// layers 2-5 in place in one buffer (51.2 KB, three workgroups per CU, a barrier per plane, the VGPR budget cut
+// to 168): 0.832 vs 0.877 ms per 16384 patches, so the extra occupancy does not pay for the barriers and the
+// register cap.
 
-// Half-plane stride (bf16 values).  Unpadded: measured on MI355X, padding the stride by 32, 64, 128 or
-// 256 bytes made the kernel 2-13% slower (more LDS bank-conflict cycles on the 16-byte A-operand reads).
-constexpr int kHalfP = kCnnPadVox * 8;
-constexpr int kBufVec = kHalfP * 2 / 8;      // uint4 per activation buffer
-constexpr int kCoarseMax = 12 * 12 * 12;     // C^3 at ghost size 5
-constexpr int kInteriorMax = 16 * 16 * 16;   // R^3 at ghost size 5
-constexpr int kFT = 256;                     // threads of a fused workgroup (4 or 8 waves)
-constexpr int kZSplit = kFT / 256;           // groups of output planes the waves split a layer into
-static_assert(kFT == 256 || kFT == 512, "4 row pairs x 1 or 2 plane groups");
+constexpr int kHalfV = S * S * 8;      // bf16 per half-plane (10 x 10 voxels x 8 channels)
+constexpr int kPlaneV = 2 * kHalfV;    // bf16 per plane
+constexpr int kActVec = So * kPlaneV / 8;   // uint4 per buffer (8 planes)
+constexpr int kCoarseMax = 12 * 12 * 12;    // C^3 at ghost size 5
+constexpr int kCubeMax = 18 * 18 * 18;      // N^3 at ghost size 5
+constexpr int kInteriorMax = 16 * 16 * 16;  // R^3 at ghost size 5
+constexpr int kFT = 256;                    // threads of a fused workgroup (4 waves)
+constexpr int kRing = 4 * S - 4;            // ring voxels of a half-plane (36)
 
-// 80.4 KB: two workgroups fit in the 160 KB of LDS of a CU
+constexpr int kFusedBlocksPerCU = 2;
+
 struct FusedLds {
-  uint4 A[kBufVec];
-  uint4 B[kBufVec];
-  union {
-    float cs[kCoarseMax];         // coarse cube, until the interpolation is done
-    float in[kInteriorMax];       // then the interpolated interior (solve cells) for the guess
-  } c;
+  uint4 X[kActVec];   // coarse staging -> packed input -> interpolated interior (FP32) for the guess
+  uint4 C[kActVec];   // interpolated cube (FP32, unpadded N^3) -> activations
+  uint4 B[kActVec];   // activations
   float red[kFT / 64];
 };
-static_assert(sizeof(FusedLds) <= 160 * 1024 / 2, "two workgroups per CU");
+static_assert(kCoarseMax * 4 <= kActVec * 16 && kInteriorMax * 4 <= kActVec * 16 && kCubeMax * 4 <= kActVec * 16,
+              "staging must fit a buffer");
+static_assert(kFusedBlocksPerCU * sizeof(FusedLds) <= 160 * 1024, "workgroups per CU");
 
 // Layer l's weights (A operand) and the biases of this lane's output channels 4*(lane/16)..+3.
 __device__ __forceinline__ void load_layer(const u16* __restrict__ Wb, const float* __restrict__ P, int l,
@@ -599,24 +607,66 @@ __device__ __forceinline__ void pair_offsets(int (&off)[5]) {
 #pragma unroll
   for (int s = 0; s < 5; s++) {
     const int pos = fwd_pos(s, g);
-    off[s] = (g & 1) * kHalfP + ((pos / 3) * S + pos % 3) * 8;
+    off[s] = (g & 1) * kHalfV + ((pos / 3) * S + pos % 3) * 8;
   }
 }
 
-// Epilogue of output plane o for row pair rp: output channels co0..co0+3 (co0 = 4*(lane/16)) of voxel
-// lane%16 of tile 4o + rp, 4 consecutive channels of one half-plane (8 bytes).
-template <int MODE>
-__device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, const floatx4& acc, const float4& b,
-                                           int o, int rp) {
-  const int lane = threadIdx.x & 63;
+// Zero the ring voxels (y or x = 0 or 9) of both half-planes of the 8 planes of a buffer.
+__device__ __forceinline__ void zero_ring(uint4* buf) {
+  for (int i = threadIdx.x; i < 2 * So * kRing; i += kFT) {
+    const int hp = i / kRing, k = i % kRing;
+    int vox;
+    if (k < S) {
+      vox = k;                                   // row 0
+    } else if (k < 2 * S) {
+      vox = (S - 1) * S + k - S;                 // row 9
+    } else {
+      const int r = k - 2 * S;                   // rows 1..8, x = 0 or 9
+      vox = (1 + (r >> 1)) * S + (r & 1) * (S - 1);
+    }
+    buf[hp * (kHalfV / 8) + vox] = make_uint4(0, 0, 0, 0);
+  }
+}
+
+// What the output layer's epilogue needs: the scale, the interpolated interior (LDS) and where the guess goes.
+struct GuessOut {
+  float s;
+  const float* in;   // interpolated interior, [R^3] FP32
+  float* guess;      // [R^3] (cube = false) or the whole solve cube [N^3] (cube = true, interior written here)
+};
+
+// Epilogue of output plane o for this wave's row pair: output channels co0..co0+3 (co0 = 4*(lane/16)) of voxel
+// lane%16 of the tile.  MODE 0/1: 4 consecutive channels of one half-plane (8 bytes), y = relu(h) (+ the
+// layer input at the same voxel, MODE 1).  MODE 2: channels 0..7 are the 2^3 sub-cells of the voxel; the
+// guess interp + s * h of those fine cells is stored.
+template <int GS, bool CUBE, int MODE>
+__device__ __forceinline__ void conv_store(const u16* in, u16* out, const floatx4& acc, const float4& b, int o,
+                                           const GuessOut& go) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int co0 = 4 * (lane >> 4);
-  const int vo = (o * So + rp + 4 * ((lane & 15) >> 3)) * So + (lane & 7);
+  const int row = wave + 4 * ((lane & 15) >> 3), xv = lane & 7;
   const float h0 = acc[0] + b.x, h1 = acc[1] + b.y, h2 = acc[2] + b.z, h3 = acc[3] + b.w;
   if (MODE == 2) {
-    if (co0 < kCnnOut) *reinterpret_cast<float4*>(D + vo * kCnnOut + co0) = make_float4(h0, h1, h2, h3);
+    if (co0 < kCnnOut) {
+      constexpr Geo g = geo_of(GS);
+      constexpr int N = g.N, R = g.R, OFF = 5 - GS;
+      const int lz = 2 * o + (co0 >> 2) - OFF, ly0 = 2 * row - OFF, lx0 = 2 * xv - OFF;
+      const float h[4] = {h0, h1, h2, h3};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int ly = ly0 + (k >> 1), lx = lx0 + (k & 1);
+        if (lz >= 0 && lz < R && ly >= 0 && ly < R && lx >= 0 && lx < R) {
+          const int t = (lz * R + ly) * R + lx;
+          const float v = go.in[t] + go.s * h[k];
+          if (CUBE)
+            go.guess[((lz + 1) * N + ly + 1) * N + lx + 1] = v;
+          else
+            go.guess[t] = v;
+        }
+      }
+    }
   } else {
-    const int q = (((vo >> 6) + 1) * S + ((vo >> 3) & 7) + 1) * S + (vo & 7) + 1;
-    const int c0 = (co0 >> 3) * kHalfP + q * 8 + (co0 & 7);
+    const int c0 = o * kPlaneV + (co0 >> 3) * kHalfV + ((row + 1) * S + xv + 1) * 8 + (co0 & 7);
     float o0 = fmaxf(h0, 0.0f), o1 = fmaxf(h1, 0.0f), o2 = fmaxf(h2, 0.0f), o3 = fmaxf(h3, 0.0f);
     if (MODE == 1) {
       const uint2 rx = *reinterpret_cast<const uint2*>(in + c0);
@@ -629,71 +679,49 @@ __device__ __forceinline__ void conv_store(const u16* in, u16* out, float* D, co
   }
 }
 
-// One convolution layer from LDS to LDS (MODE 0/1) or to the FP32 output D in LDS (MODE 2).  Wave w owns
-// output rows y = 2rp, 2rp+1 (rp = w % 4) of the NO = 8 / kZSplit output planes P0.. of its plane group and
-// sweeps the NO + 2 input planes they read once: input plane P0 + q feeds output planes P0 + q (dz = 0),
-// P0 + q - 1 (dz = 1) and P0 + q - 2 (dz = 2), so every A operand read from LDS serves up to three MFMAs
-// (with 4 waves: 50 reads for 120 MFMAs per wave instead of one read per MFMA).  Output plane o still
-// accumulates its 15 steps in the order of conv_fwd_kernel: dz = 0, 1, 2, and s = 0..4 within each.
-template <int MODE>
-__device__ __forceinline__ void conv_lds(const u16* in, u16* out, const bf16x8 (&wreg)[KZ], const float4& b, float* D,
-                                         const int (&off)[5]) {
-  constexpr int NO = So / kZSplit, NQ = NO + 2;
+// One convolution layer (in != out).  Input plane z (0..7) feeds output plane z + 1 (dz = 0), z (dz = 1) and
+// z - 1 (dz = 2); output plane o accumulates its steps in the order of conv_fwd_kernel (dz = 0, 1, 2; s = 0..4
+// within each) without the steps of the zero planes z = -1 and 8.  The operands of plane z + 1 are requested
+// before the MFMAs of plane z, so their LDS latency hides behind those.
+template <int GS, bool CUBE, int MODE>
+__device__ __forceinline__ void conv_layer(const u16* in, u16* out, const bf16x8 (&w)[KZ], const float4& b,
+                                           const int (&off)[5], const GuessOut& go) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15;
-  const int rp = wave & 3, P0 = (wave >> 2) * NO;
-  const u16* col = in + ((P0 * S + rp + 4 * (i >> 3)) * S + (i & 7)) * 8;   // plane P0, row voxel of lane i
+  const u16* col = in + ((wave + 4 * (i >> 3)) * S + (i & 7)) * 8;   // padded row = output row + dy
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
-  floatx4 a0 = zero, a1 = zero;   // output planes P0+q-2 and P0+q-1
+  floatx4 a0 = zero, a1 = zero;   // output planes z - 1 and z
+  Pack8 nx[5];
 #pragma unroll
-  for (int q = 0; q < NQ; q++) {
+  for (int s = 0; s < 5; s++) nx[s].u = *reinterpret_cast<const uint4*>(col + off[s]);
+#pragma unroll
+  for (int z = 0; z < So; z++) {
     Pack8 a[5];
 #pragma unroll
-    for (int s = 0; s < 5; s++) a[s].u = *reinterpret_cast<const uint4*>(col + q * (S * S * 8) + off[s]);
-    floatx4 a2 = zero;            // output plane P0+q
+    for (int s = 0; s < 5; s++) a[s] = nx[s];
+    if (z + 1 < So) {
+#pragma unroll
+      for (int s = 0; s < 5; s++) nx[s].u = *reinterpret_cast<const uint4*>(col + (z + 1) * kPlaneV + off[s]);
+    }
+    floatx4 a2 = zero;            // output plane z + 1
 #pragma unroll
     for (int s = 0; s < 5; s++) {
-      if (q < NO) a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[s], a[s].v, a2, 0, 0, 0);
-      if (q >= 1 && q <= NO) a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[5 + s], a[s].v, a1, 0, 0, 0);
-      if (q >= 2) a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[10 + s], a[s].v, a0, 0, 0, 0);
+      if (z + 1 < So) a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[s], a[s].v, a2, 0, 0, 0);
+      a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[5 + s], a[s].v, a1, 0, 0, 0);
+      if (z >= 1) a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[10 + s], a[s].v, a0, 0, 0, 0);
     }
-    if (q >= 2) conv_store<MODE>(in, out, D, a0, b, P0 + q - 2, rp);
+    if (z >= 1) conv_store<GS, CUBE, MODE>(in, out, a0, b, z - 1, go);
     a0 = a1;
     a1 = a2;
   }
+  conv_store<GS, CUBE, MODE>(in, out, a0, b, So - 1, go);   // its dz = 2 input is the zero plane z = 8
 }
 
-// Zero the border voxels (any coordinate 0 or 9) of both half-planes of a padded activation buffer.  The
-// 488 border voxels of a half-plane are enumerated directly: the planes z = 0 and z = 9 (100 each), then per
-// inner plane the rows y = 0 and y = 9 (10 each) and the x = 0 / x = 9 ends of the 8 inner rows.
-__device__ __forceinline__ void zero_border(uint4* buf) {
-  constexpr int SS = S * S, RING = 4 * S - 4, NB = 2 * SS + (S - 2) * RING;
-  static_assert(NB == kCnnPadVox - (S - 2) * (S - 2) * (S - 2), "border count");
-  for (int i = threadIdx.x; i < 2 * NB; i += kFT) {
-    const int h = i / NB;
-    int k = i % NB, vox;
-    if (k < 2 * SS) {
-      vox = k < SS ? k : k + (S - 2) * SS;
-    } else {
-      k -= 2 * SS;
-      const int z = 1 + k / RING, r = k % RING;
-      if (r < 2 * S)
-        vox = z * SS + (r < S ? r : (S - 1) * S + r - S);
-      else
-        vox = z * SS + (1 + (r - 2 * S) / 2) * S + ((r - 2 * S) & 1) * (S - 1);
-    }
-    buf[h * (kHalfP / 8) + vox] = make_uint4(0, 0, 0, 0);
-  }
-}
-
-// Interpolated solve cube from a coarse cube already staged in LDS (same arithmetic as interp_cube), into a
-// cube stored with pitch N + 2 and one spare cell on each side (cube cell (X, Y, Z) at
-// cube[(Z * (N + 2) + Y) * (N + 2) + X], -1 <= X, Y, Z <= N), so that the children that fall outside the solve
-// cube land in the spare layer and every child is stored without a test.  Returns this thread's max |value|
-// over the children inside the solve cube (the cube's max once reduced over the workgroup).
+// Interpolated solve cube from a coarse cube staged in LDS (the arithmetic of interp_cube) into an unpadded
+// N^3 cube.  Returns this thread's max |value| (the cube's max once reduced over the workgroup).
 template <int GS>
-__device__ float interp_cube_lds(const float* cs, int int_scheme, float* cube) {
+__device__ __forceinline__ float interp_cube_lds(const float* cs, int int_scheme, float* cube) {
   constexpr Geo g = geo_of(GS);
-  constexpr int N = g.N, NP = N + 2, C = g.C, U = g.U, NC = C - 2;
+  constexpr int N = g.N, C = g.C, U = g.U, NC = C - 2;
   float mx = 0.0f;
   for (int q = threadIdx.x; q < NC * NC * NC; q += kFT) {
     const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
@@ -704,202 +732,175 @@ __device__ float interp_cube_lds(const float* cs, int int_scheme, float* cube) {
     float ch[2][2][2];
     interp_children(v, int_scheme, ch);
     const int X0 = 2 * (i - 1) - U, Y0 = 2 * (j - 1) - U, Z0 = 2 * (k - 1) - U;
-    const bool vx[2] = {X0 >= 0, X0 + 1 < N}, vy[2] = {Y0 >= 0, Y0 + 1 < N}, vz[2] = {Z0 >= 0, Z0 + 1 < N};
-    float* base = cube + (Z0 * NP + Y0) * NP + X0;
 #pragma unroll
     for (int cz = 0; cz < 2; cz++)
 #pragma unroll
       for (int cy = 0; cy < 2; cy++)
 #pragma unroll
         for (int cx = 0; cx < 2; cx++) {
-          base[(cz * NP + cy) * NP + cx] = ch[cz][cy][cx];
-          mx = fmaxf(mx, vx[cx] && vy[cy] && vz[cz] ? fabsf(ch[cz][cy][cx]) : 0.0f);
+          const int X = X0 + cx, Y = Y0 + cy, Z = Z0 + cz;
+          if (X >= 0 && X < N && Y >= 0 && Y < N && Z >= 0 && Z < N) {
+            cube[(Z * N + Y) * N + X] = ch[cz][cy][cx];
+            mx = fmaxf(mx, fabsf(ch[cz][cy][cx]));
+          }
         }
   }
   return mx;
 }
 
-template <int GS>
+template <int GS, bool CUBE>
 __device__ void fused_patch(const float* __restrict__ rho, const float* __restrict__ cpot, const u16* __restrict__ Wb,
                             const float* __restrict__ P, float cst, int int_scheme, float* __restrict__ guess,
                             FusedLds& L) {
   constexpr Geo g = geo_of(GS);
   constexpr int N = g.N, R = g.R, C = g.C, OFF = 5 - GS;
-  constexpr int R3 = R * R * R, C3 = C * C * C;
-  static_assert(N * N * N * 4 <= kBufVec * 16 && kCnnOutputs * 4 <= kBufVec * 16, "scratch must fit a buffer");
-  static_assert(C3 <= kCoarseMax && R3 <= kInteriorMax && R3 % 4 == 0 && C3 % 4 == 0, "float4 staging");
+  constexpr int C3 = C * C * C;
+  static_assert(C3 % 4 == 0, "float4 staging");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int NP = N + 2;   // cube pitch (one spare cell on each side)
-  static_assert(NP * NP * NP * 4 <= kBufVec * 16, "padded cube must fit a buffer");
-  float* cube = reinterpret_cast<float*>(L.B) + (NP * NP + NP + 1);   // cube cell (0, 0, 0)
-  u16* A = reinterpret_cast<u16*>(L.A);
-  u16* B = reinterpret_cast<u16*>(L.B);
-  // first layer's weights, the coarse cube and this lane's share of the density are requested together
-  bf16x8 wa[KZ], wb[KZ];
-  float4 ba, bb;
-  load_layer(Wb, P, 0, wa, ba);
+  float* cs = reinterpret_cast<float*>(L.X);      // coarse staging
+  float* cube = reinterpret_cast<float*>(L.C);
+  u16* X0 = reinterpret_cast<u16*>(L.X);
+  u16* A = reinterpret_cast<u16*>(L.C);
+  // first layer's weights, the coarse cube and this thread's density column are requested together.  Thread t
+  // owns the frame-interior column (fx, fy) = (t % 16, t / 16); its solve-interior column is (fx, fy) - OFF.
+  bf16x8 w[KZ];
+  float4 b;
+  load_layer(Wb, P, 0, w, b);
   for (int t = threadIdx.x; t < C3 / 4; t += kFT)
-    reinterpret_cast<float4*>(L.c.cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
-  // Thread t < R^2 owns the solve-interior column (x, y) = (t % R, t / R) for the packing and the guess; its
-  // density column is requested now (each wavefront load reads 64 consecutive floats).
-  constexpr int RR = R * R;
-  static_assert(RR <= kFT, "one thread per interior column");
-  const bool owns_col = threadIdx.x < RR;
+    reinterpret_cast<float4*>(cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
+  const int fx = threadIdx.x & 15, fy = threadIdx.x >> 4;
+  const int x = fx - OFF, y = fy - OFF;
+  const bool colv = x >= 0 && x < R && y >= 0 && y < R;
   float rcol[R];
 #pragma unroll
-  for (int z = 0; z < R; z++) rcol[z] = owns_col ? rho[z * RR + threadIdx.x] : 0.0f;
+  for (int z = 0; z < R; z++) rcol[z] = colv ? rho[(z * R + y) * R + x] : 0.0f;
   __syncthreads();
-  float mx = interp_cube_lds<GS>(L.c.cs, int_scheme, cube);
+  float mx = interp_cube_lds<GS>(cs, int_scheme, cube);
   for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
   if (lane == 0) L.red[wave] = mx;
   __syncthreads();   // the cube is complete and the coarse staging is dead
   mx = L.red[0];
 #pragma unroll
-  for (int w = 1; w < kFT / 64; w++) mx = fmaxf(mx, L.red[w]);
+  for (int k = 1; k < kFT / 64; k++) mx = fmaxf(mx, L.red[k]);
   const float s = mx > 0.0f ? mx : 1.0f;
   const float inv = 1.0f / s;
-  // Packed input over the 20^3 frame (frame cell e = cube cell + 1 + OFF): channel pair (q0, q1) of sub-cell
-  // d of voxel e/2 is one 32-bit word (q0 low) at half-plane d/4, in-half channels 2(d%4), 2(d%4)+1.
-  // Solve-interior cells get q0 = interp/s and q1 = 64 r/s (r from the cube, still in B, and the density);
-  // the other frame cells get q0 = interp/s on the cube's Dirichlet layer, 0 outside the cube, and q1 = 0.
-  // The column owners sweep z with the column's cube values in registers; the frame columns outside the
-  // interior are spread over all threads.  Both write disjoint words, so they share one barrier.
-  uint32_t* A32 = reinterpret_cast<uint32_t*>(A);
-  constexpr int HW = kHalfP / 2;                 // 32-bit words per half-plane
-  constexpr int E = 2 * S, LO = 2 + OFF, HI = LO + R;   // frame edge, interior frame range [LO, HI)
-  if (owns_col) {
-    const int x = threadIdx.x % R, y = threadIdx.x / R;
-    const int ex = x + LO, ey = y + LO;
-    const int colw = (((ey >> 1) * S + (ex >> 1)) * 4) + (ey & 1) * 2 + (ex & 1);   // word of frame z = 0
-    const float* cc = cube + (y + 1) * NP + (x + 1);                              // cube column, stride NP^2
-#pragma unroll
-    for (int ez = 0; ez < LO; ez++) {   // below the interior: Dirichlet layer, then outside the cube
-      const int Z = ez - 1 - OFF;
-      const float c0 = Z >= 0 ? cc[Z * NP * NP] * inv : 0.0f;
-      A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = (uint32_t)f2bf(c0);
+  // The whole-cube guess takes the Dirichlet layer from the interpolation (its interior is the epilogue's).
+  if (CUBE) {
+    constexpr int NSHELL = 2 * N * N + 4 * (N - 1) * (N - 2);
+    for (int q = threadIdx.x; q < NSHELL; q += kFT) {
+      int X, Y, Z;
+      shell_cell<N>(q, X, Y, Z);
+      const int t = (Z * N + Y) * N + X;
+      guess[t] = cube[t];
     }
+  }
+  // Packed input on the 16^3 frame interior (the solve interior, centred; zero elsewhere): fine cell
+  // (fx, fy, fz) is sub-cell ((fz&1)*2 + (fy&1))*2 + (fx&1) of voxel (fx, fy, fz)/2, i.e. half-plane fz&1 and
+  // 32-bit word (fy&1)*2 + (fx&1) of the voxel; the word is (q0, q1) = (interp/s, 64 r/s) with r the SOR
+  // residual of the interpolated cube.  The column owner sweeps z with the column's cube values in registers
+  // and keeps the column's interpolated values for the guess.
+  uint32_t* X32 = reinterpret_cast<uint32_t*>(X0);
+  const int colw = (((fy >> 1) + 1) * S + (fx >> 1) + 1) * 4 + (fy & 1) * 2 + (fx & 1);
+  auto word = [&](int fz) -> uint32_t& { return X32[(fz >> 1) * (kPlaneV / 2) + (fz & 1) * (kHalfV / 2) + colw]; };
+  float icol[R];
 #pragma unroll
-    for (int ez = HI; ez < E; ez++) {   // above the interior
-      const int Z = ez - 1 - OFF;
-      const float c0 = Z < N ? cc[Z * NP * NP] * inv : 0.0f;
-      A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = (uint32_t)f2bf(c0);
-    }
-    float cm = cc[0], c = cc[NP * NP];
+  for (int fz = 0; fz < OFF; fz++) word(fz) = 0u;
+#pragma unroll
+  for (int fz = OFF + R; fz < 16; fz++) word(fz) = 0u;
+  if (colv) {
+    const float* cc = cube + (y + 1) * N + (x + 1);   // cube column, stride N^2
+    float cm = cc[0], c = cc[N * N];
 #pragma unroll
     for (int z = 0; z < R; z++) {
-      const float* cz = cc + (z + 1) * NP * NP;
-      const float cp = cz[NP * NP];
-      const float r = cp + cm + cz[NP] + cz[-NP] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
-      const int ez = z + LO;
-      A32[(ez & 1) * HW + (ez >> 1) * (S * S * 4) + colw] = pack2_bf16(c * inv, (r * inv) * kResScale);
-      L.c.in[(z * R + y) * R + x] = c;
+      const float* cz = cc + (z + 1) * N * N;
+      const float cp = cz[N * N];
+      const float r = cp + cm + cz[N] + cz[-N] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
+      word(z + OFF) = pack2_bf16(c * inv, (r * inv) * kResScale);
+      icol[z] = c;
       cm = c;
       c = cp;
     }
-  }
-  // Frame columns outside the interior (rows below LO and above HI, then the sides of the interior rows),
-  // as half-columns of E/2 cells: item i is column i/2, cells ez in [(i&1)*E/2, (i&1)*E/2 + E/2).
-  constexpr int NOC = E * E - RR;
-  for (int i = threadIdx.x; i < 2 * NOC; i += kFT) {
-    const int j = i >> 1, z0 = (i & 1) * (E / 2);
-    int ex, ey;
-    if (j < LO * E) {
-      ey = j / E;
-      ex = j % E;
-    } else if (j < E * E - HI * E + LO * E) {
-      ey = HI + (j - LO * E) / E;
-      ex = (j - LO * E) % E;
-    } else {
-      const int k = j - (E * E - HI * E + LO * E), w = E - R;
-      ey = LO + k / w;
-      ex = k % w < LO ? k % w : k % w - LO + HI;
-    }
-    const int X = ex - 1 - OFF, Y = ey - 1 - OFF;
-    const bool xy_in = X >= 0 && X < N && Y >= 0 && Y < N;
-    const float* cc = cube + (xy_in ? Y * NP + X : 0);
-    uint32_t* w0 = A32 + ((ey >> 1) * S + (ex >> 1)) * 4 + (ey & 1) * 2 + (ex & 1);
+  } else {
 #pragma unroll
-    for (int dz = 0; dz < E / 2; dz++) {
-      const int ez = z0 + dz, Z = ez - 1 - OFF;
-      const float c0 = xy_in && Z >= 0 && Z < N ? cc[Z * NP * NP] * inv : 0.0f;
-      w0[(ez & 1) * HW + (ez >> 1) * (S * S * 4)] = (uint32_t)f2bf(c0);
+    for (int z = 0; z < R; z++) {
+      word(z + OFF) = 0u;
+      icol[z] = 0.0f;
     }
   }
-  __syncthreads();   // every lane is done reading the cube in B
-  for (int i = threadIdx.x; i < kBufVec; i += kFT) L.B[i] = make_uint4(0, 0, 0, 0);
+  zero_ring(L.X);
+  zero_ring(L.B);
+  __syncthreads();   // the packed input is complete; the cube is dead
+  zero_ring(L.C);
   int off[5];
   pair_offsets(off);
   __syncthreads();
-  load_layer(Wb, P, 1, wb, bb);
-  conv_lds<0>(A, B, wa, ba, nullptr, off);     // layer 1: A -> B
-  __syncthreads();
-  zero_border(L.A);                            // A held the input's border
-  load_layer(Wb, P, 2, wa, ba);
-  conv_lds<1>(B, A, wb, bb, nullptr, off);     // layer 2: B -> A
-  __syncthreads();
-  load_layer(Wb, P, 3, wb, bb);
-  conv_lds<1>(A, B, wa, ba, nullptr, off);     // layer 3
-  __syncthreads();
-  load_layer(Wb, P, 4, wa, ba);
-  conv_lds<1>(B, A, wb, bb, nullptr, off);     // layer 4
-  __syncthreads();
-  load_layer(Wb, P, 5, wb, bb);
-  conv_lds<1>(A, B, wa, ba, nullptr, off);     // layer 5
-  __syncthreads();
-  float* D = reinterpret_cast<float*>(L.A);
-  conv_lds<2>(B, nullptr, wb, bb, D, off);     // layer 6: B -> D (in A)
-  __syncthreads();
-  if (owns_col) {   // the guess, column by column (each wavefront store covers 64 consecutive floats)
-    const int x = threadIdx.x % R, y = threadIdx.x / R;
-    const int ix = x + OFF, iy = y + OFF;
-    const int colo = ((iy >> 1) * So + (ix >> 1)) * kCnnOut + (iy & 1) * 2 + (ix & 1);
+  GuessOut go{s, reinterpret_cast<const float*>(L.X), guess};
+  conv_layer<GS, CUBE, 0>(X0, A, w, b, off, go);   // layer 1: packed input -> A
+  __syncthreads();   // every wave is done reading the packed input
+  if (colv) {
+    float* in = reinterpret_cast<float*>(L.X);
 #pragma unroll
-    for (int z = 0; z < R; z++) {
-      const int iz = z + OFF, t = (z * R + y) * R + x;
-      guess[t] = L.c.in[t] + s * D[colo + (iz >> 1) * (So * So * kCnnOut) + (iz & 1) * 4];
-    }
+    for (int z = 0; z < R; z++) in[(z * R + y) * R + x] = icol[z];
   }
+  u16* Bb = reinterpret_cast<u16*>(L.B);
+  load_layer(Wb, P, 1, w, b);
+  conv_layer<GS, CUBE, 1>(A, Bb, w, b, off, go);   // layers 2-5: A -> B -> A -> B -> A
+  __syncthreads();
+  load_layer(Wb, P, 2, w, b);
+  conv_layer<GS, CUBE, 1>(Bb, A, w, b, off, go);
+  __syncthreads();
+  load_layer(Wb, P, 3, w, b);
+  conv_layer<GS, CUBE, 1>(A, Bb, w, b, off, go);
+  __syncthreads();
+  load_layer(Wb, P, 4, w, b);
+  conv_layer<GS, CUBE, 1>(Bb, A, w, b, off, go);
+  __syncthreads();
+  load_layer(Wb, P, 5, w, b);
+  conv_layer<GS, CUBE, 2>(A, nullptr, w, b, off, go);   // layer 6: the guess
 }
 
+template <bool CUBE>
 __device__ __forceinline__ void fused_dispatch(int gs, const float* r, const float* c, const u16* Wb, const float* P,
                                                float cst, int int_scheme, float* out, FusedLds& L) {
   switch (gs) {
-    case 1: fused_patch<1>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 2: fused_patch<2>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 3: fused_patch<3>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 4: fused_patch<4>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    default: fused_patch<5>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 1: fused_patch<1, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 2: fused_patch<2, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 3: fused_patch<3, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 4: fused_patch<4, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    default: fused_patch<5, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
   }
 }
 
-__global__ void __launch_bounds__(kFT, kFT / 128) fused_guess_kernel(const float* __restrict__ rho,
-                                                               const float* __restrict__ cpot,
-                                                               const u16* __restrict__ Wb, const float* __restrict__ P,
-                                                               int npatch, int gs, float cst, int int_scheme,
-                                                               float* __restrict__ guess) {
+// CUBE: the guess is the whole solve cube [n][N^3] (Dirichlet layer included), else the interior [n][R^3].
+template <bool CUBE>
+__global__ void __launch_bounds__(kFT, kFusedBlocksPerCU) fused_guess_kernel(const float* __restrict__ rho,
+                                                             const float* __restrict__ cpot,
+                                                             const u16* __restrict__ Wb, const float* __restrict__ P,
+                                                             int npatch, int gs, float cst, int int_scheme,
+                                                             float* __restrict__ guess) {
   __shared__ FusedLds L;
   const int p = blockIdx.x;
   if (p >= npatch) return;
   const Geo g = geo_of(gs);
   const float* r = rho + (size_t)p * g.R * g.R * g.R;
   const float* c = cpot + (size_t)p * g.C * g.C * g.C;
-  float* out = guess + (size_t)p * g.R * g.R * g.R;
-  fused_dispatch(gs, r, c, Wb, P, cst, int_scheme, out, L);
+  float* out = guess + (size_t)p * (CUBE ? g.N * g.N * g.N : g.R * g.R * g.R);
+  fused_dispatch<CUBE>(gs, r, c, Wb, P, cst, int_scheme, out, L);
 }
 
 // Mixed ghost sizes: patch p reads its ghost size and packed offsets from the plan's descriptors; the
 // guess is packed like the density (offset rho_off).
-__global__ void __launch_bounds__(kFT, kFT / 128) fused_guess_mixed_kernel(const float* __restrict__ rho,
-                                                                     const float* __restrict__ cpot,
-                                                                     const PatchDesc* __restrict__ desc,
-                                                                     const u16* __restrict__ Wb,
-                                                                     const float* __restrict__ P, int npatch, float cst,
-                                                                     int int_scheme, float* __restrict__ guess) {
+__global__ void __launch_bounds__(kFT, kFusedBlocksPerCU) fused_guess_mixed_kernel(const float* __restrict__ rho,
+                                                                   const float* __restrict__ cpot,
+                                                                   const PatchDesc* __restrict__ desc,
+                                                                   const u16* __restrict__ Wb,
+                                                                   const float* __restrict__ P, int npatch, float cst,
+                                                                   int int_scheme, float* __restrict__ guess) {
   __shared__ FusedLds L;
   const int p = blockIdx.x;
   if (p >= npatch) return;
   const PatchDesc d = desc[p];
-  fused_dispatch(d.gs, rho + d.rho_off, cpot + d.cpot_off, Wb, P, cst, int_scheme, guess + d.rho_off, L);
+  fused_dispatch<false>(d.gs, rho + d.rho_off, cpot + d.cpot_off, Wb, P, cst, int_scheme, guess + d.rho_off, L);
 }
 
 }  // namespace cnn
@@ -978,9 +979,13 @@ hipError_t cnn_prep(const float* rho, const float* cpot, int n, int gs, float cs
 }
 
 hipError_t cnn_fused_guess(const float* rho, const float* cpot, const uint16_t* Wb, const float* P, int n, int gs,
-                           float cst, int int_scheme, float* guess, hipStream_t s) {
-  hipLaunchKernelGGL(fused_guess_kernel, dim3(n), dim3(kFT), 0, s, rho, cpot, Wb, P, n, gs, cst, int_scheme,
-                     guess);
+                           float cst, int int_scheme, float* guess, int cube, hipStream_t s) {
+  if (cube)
+    hipLaunchKernelGGL(fused_guess_kernel<true>, dim3(n), dim3(kFT), 0, s, rho, cpot, Wb, P, n, gs, cst, int_scheme,
+                       guess);
+  else
+    hipLaunchKernelGGL(fused_guess_kernel<false>, dim3(n), dim3(kFT), 0, s, rho, cpot, Wb, P, n, gs, cst, int_scheme,
+                       guess);
   return hipGetLastError();
 }
 

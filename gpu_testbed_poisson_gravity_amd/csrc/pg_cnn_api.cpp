@@ -322,7 +322,20 @@ int pgamd_cnn_warm_start(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho, c
   hipSetDevice(n->device);
   const float cst = a->poi_coeff * a->dh * a->dh;
   // one launch: interpolation, packing, the six layers in LDS and the guess
-  CK(cnn_fused_guess(d_rho, d_pot_in, n->wb, n->p, npatch, gs, cst, a->int_scheme, d_guess, (hipStream_t)stream));
+  CK(cnn_fused_guess(d_rho, d_pot_in, n->wb, n->p, npatch, gs, cst, a->int_scheme, d_guess, 0, (hipStream_t)stream));
+  return PGAMD_OK;
+}
+
+int pgamd_cnn_warm_start_cube(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho, const float* d_pot_in, int npatch,
+                              const pgamd_solver_args* a, float* d_guess, void* stream) {
+  if (!ctx || !n || !a || !d_rho || !d_pot_in || !d_guess || npatch <= 0)
+    return set_error(PGAMD_ERR_ARG, "bad warm-start arguments");
+  if (a->int_scheme != PGAMD_INT_CQUAD && a->int_scheme != PGAMD_INT_QUAD)
+    return set_error(PGAMD_ERR_ARG, "incorrect parameter IntScheme");
+  const int gs = ctx_options(ctx).pot_ghost_size;
+  hipSetDevice(n->device);
+  const float cst = a->poi_coeff * a->dh * a->dh;
+  CK(cnn_fused_guess(d_rho, d_pot_in, n->wb, n->p, npatch, gs, cst, a->int_scheme, d_guess, 1, (hipStream_t)stream));
   return PGAMD_OK;
 }
 
@@ -374,7 +387,7 @@ int pgamd_cnn_solve_device(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho,
   for (int i = 0, off = 0; off < npatch; i++, off += chunk) {
     const int m = std::min(chunk, npatch - off);
     CK(cnn_fused_guess(d_rho + off * R3, d_pot_in + off * C3, n->wb, n->p, m, o.pot_ghost_size, cst, a->int_scheme,
-                       d_guess + off * R3, s1));
+                       d_guess + off * R3, 0, s1));
     CK(hipEventRecord(n->ev[i], s1));
     CK(hipStreamWaitEvent(s0, n->ev[i], 0));
     const int rc = pgamd_solve_device(ctx, d_rho + off * R3, d_pot_in + off * C3, d_guess + off * R3,

@@ -12,6 +12,7 @@ namespace pgamd {
 // way the reference CPU solver forms them (CPU_PoissonSolver_SOR.cpp:51-52), so that device and CPU agree
 // bit for bit.
 struct SolveParams {
+  using real = float;
   float cst;          // Poi_Coeff*dh*dh
   float w6;           // omega/6
   float damp;         // |1 - omega|, second-colour residual factor of the relative-residual rule
@@ -20,7 +21,8 @@ struct SolveParams {
   int max_iter;
   int term_mode;      // 0: reference stall rule; 1: relative residual
   int check_every;    // relative-residual cadence
-  int init_mode;      // 0: interpolated cube; 1: zero interior; 2: caller guess
+  int init_mode;      // 0: interpolated cube; 1: zero interior; 2: caller guess of the interior [R^3];
+                      // 3: caller guess of the whole cube [N^3] (SOR also takes its Dirichlet layer)
   int int_scheme;     // 4 CQUAD, 5 QUAD
   int gra_ghost;      // GRA_GHOST_SIZE of the output window (1..gs; > 2 only without gravity)
   // multigrid
@@ -34,9 +36,36 @@ struct SolveParams {
   int fuse_gravity;
   int p5;
   float gra_const;
-  // initial-guess check (init_mode 2): a guess whose relative residual exceeds guess_tol is replaced by
+  // initial-guess check (init_mode 2, 3): a guess whose relative residual exceeds guess_tol is replaced by
   // the zero guess before the first sweep; 0 disables the check
   float guess_tol;
+};
+
+// FP64 settings (the reference's FLOAT8 build, Makefile:22, Header/Macro.h:35): the same fields, with every
+// floating-point constant formed in double on the host.
+struct SolveParams64 {
+  using real = double;
+  double cst;
+  double w6;
+  double damp;
+  double rel_tol;
+  int min_iter;
+  int max_iter;
+  int term_mode;
+  int check_every;
+  int init_mode;
+  int int_scheme;
+  int gra_ghost;
+  double poi_coeff;
+  double dh;
+  double mg_tol;
+  int mg_max_iter;
+  int mg_npre;
+  int mg_npost;
+  int fuse_gravity;
+  int p5;
+  double gra_const;
+  double guess_tol;
 };
 
 // Status bits written per patch.
@@ -45,19 +74,23 @@ constexpr int kStatusMaxIter = 2;    // max_iter reached
 constexpr int kStatusNonFinite = 4;  // a non-finite value is present in the returned potential
 constexpr int kStatusFallback = 8;   // the caller's guess failed the residual check; the zero guess was used
 
-// One batch of patches sharing a ghost size, with dense per-patch strides (the reference layout).
-struct UniformBatch {
-  const float* rho;    // [n][R^3]
-  const float* cpot;   // [n][C^3]
-  const float* guess;  // [n][R^3] or null
-  float* pot_out;      // [n][G^3]
-  float* flu;          // [n][5][8^3] or null
-  int* iters;          // [n] or null
-  int* status;         // [n] or null
+// One batch of patches sharing a ghost size, with dense per-patch strides (the reference layout); T is the
+// element type (float, or double for the FP64 kernels).
+template <typename T>
+struct UniformBatchT {
+  const T* rho;    // [n][R^3]
+  const T* cpot;   // [n][C^3]
+  const T* guess;  // [n][R^3] (init_mode 2), [n][N^3] (init_mode 3) or null
+  T* pot_out;      // [n][G^3]
+  T* flu;          // [n][5][8^3] or null
+  int* iters;      // [n] or null
+  int* status;     // [n] or null
   int npatch;
   int gs;
   int scheme;
 };
+using UniformBatch = UniformBatchT<float>;
+using UniformBatch64 = UniformBatchT<double>;
 
 // A patch of a mixed batch: its ghost size, its method and element offsets into packed arrays.
 struct PatchDesc {
@@ -129,5 +162,10 @@ int default_pad(int gs);
 size_t lds_bytes_sor(int gs, int ppw, int pad);
 size_t lds_bytes_mg(int gs);
 int supported_ppw(int gs, int ppw);
+// FP64 kernels (pg_f64.hip and the FP64 instances of the MG kernel): one workgroup per patch.
+hipError_t launch_uniform64(const UniformBatch64& b, const SolveParams64& p, hipStream_t stream);
+hipError_t launch_gravity64(const double* pot, double* flu, int npatch, int gra_ghost, double gra_const, int p5,
+                            hipStream_t stream);
+hipError_t launch_mg_uniform64(const UniformBatch64& b, const SolveParams64& p, hipStream_t s);
 
 }  // namespace pgamd

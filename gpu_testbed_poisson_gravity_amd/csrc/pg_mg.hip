@@ -70,7 +70,8 @@ constexpr int kMgThreads = 256;
 __device__ __forceinline__ int nat(int n, int x, int y, int z) { return (z * n + y) * n + x; }
 
 // Block-wide sums of two values in a fixed order (bitwise reproducible for a given launch shape).
-__device__ __forceinline__ void block_sum2(float& a, float& b, float* red) {
+template <typename T>
+__device__ __forceinline__ void block_sum2(T& a, T& b, T* red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off, 64);
@@ -98,11 +99,11 @@ __device__ __forceinline__ void cell0(int q, int c, int& x, int& y, int& z) {
 }
 
 // Red-black Gauss-Seidel pass of one colour at level l (CPU_PoissonSolver_MG.cpp:392-441).
-template <int GS>
-__device__ void smooth_level0(float* __restrict__ sol, const float (&rhs)[2][MgGeo<GS>::PER_THREAD], float dh2) {
+template <int GS, typename T>
+__device__ void smooth_level0(T* __restrict__ sol, const T (&rhs)[2][MgGeo<GS>::PER_THREAD], T dh2) {
   using M = MgGeo<GS>;
   constexpr int N = M::N;
-  const float one_six = 1.0f / 6.0f;
+  const T one_six = T(1) / T(6);
 #pragma unroll
   for (int pass = 0; pass < 2; pass++) {
     const int c = 1 - pass;
@@ -121,8 +122,9 @@ __device__ void smooth_level0(float* __restrict__ sol, const float (&rhs)[2][MgG
   }
 }
 
-__device__ void smooth_coarse(float* __restrict__ sol, const float* __restrict__ rhs, float dh2, int n) {
-  const float one_six = 1.0f / 6.0f;
+template <typename T>
+__device__ void smooth_coarse(T* __restrict__ sol, const T* __restrict__ rhs, T dh2, int n) {
+  const T one_six = T(1) / T(6);
   const int m = n - 2;
   for (int pass = 0; pass < 2; pass++) {
     const int c = 1 - pass;
@@ -139,13 +141,13 @@ __device__ void smooth_coarse(float* __restrict__ sol, const float* __restrict__
 
 // Defect -(lap(sol) - rhs) on the level-0 interior, with the L1 error estimate when requested
 // (CPU_PoissonSolver_MG.cpp:460-514).
-template <int GS>
-__device__ float defect_level0(const float* __restrict__ sol, float* __restrict__ def,
-                               const float (&rhs)[2][MgGeo<GS>::PER_THREAD], float dh, bool estimate, float* red) {
+template <int GS, typename T>
+__device__ T defect_level0(const T* __restrict__ sol, T* __restrict__ def, const T (&rhs)[2][MgGeo<GS>::PER_THREAD],
+                           T dh, bool estimate, T* red) {
   using M = MgGeo<GS>;
   constexpr int N = M::N;
-  const float m_dh2 = -1.0f / (dh * dh);
-  float sd = 0.0f, ss = 0.0f;
+  const T m_dh2 = T(-1) / (dh * dh);
+  T sd = T(0), ss = T(0);
 #pragma unroll
   for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -155,52 +157,53 @@ __device__ float defect_level0(const float* __restrict__ sol, float* __restrict_
         int x, y, z;
         cell0<GS>(q, c, x, y, z);
         const int id = nat(N, x, y, z);
-        const float d = m_dh2 * (sol[id + N * N] + sol[id - N * N] + sol[id + N] + sol[id - N] + sol[id + 1] +
-                                 sol[id - 1] - 6.0f * sol[id]) +
-                        rhs[c][s];
+        const T d = m_dh2 * (sol[id + N * N] + sol[id - N * N] + sol[id + N] + sol[id - N] + sol[id + 1] +
+                             sol[id - 1] - T(6) * sol[id]) +
+                    rhs[c][s];
         def[id] = d;
-        sd += fabsf(d);
-        ss += fabsf(sol[id]);
+        sd += fabs(d);
+        ss += fabs(sol[id]);
       }
     }
   __syncthreads();
-  if (!estimate) return 0.0f;
+  if (!estimate) return T(0);
   block_sum2(sd, ss, red);
   return dh * dh * sd / ss;
 }
 
-__device__ void defect_coarse(const float* __restrict__ sol, const float* __restrict__ rhs, float* __restrict__ def,
-                              float dh, int n) {
-  const float m_dh2 = -1.0f / (dh * dh);
+template <typename T>
+__device__ void defect_coarse(const T* __restrict__ sol, const T* __restrict__ rhs, T* __restrict__ def, T dh, int n) {
+  const T m_dh2 = T(-1) / (dh * dh);
   const int m = n - 2;
   for (int q = threadIdx.x; q < m * m * m; q += kMgThreads) {
     const int x = q % m + 1, y = (q / m) % m + 1, z = q / (m * m) + 1;
     const int id = nat(n, x, y, z);
     def[id] = m_dh2 * (sol[id + n * n] + sol[id - n * n] + sol[id + n] + sol[id - n] + sol[id + 1] + sol[id - 1] -
-                       6.0f * sol[id]) +
+                       T(6) * sol[id]) +
               rhs[id];
   }
   __syncthreads();
 }
 
 // 27-point restriction with ratio (nf-1)/(nc-1) (CPU_PoissonSolver_MG.cpp:533-616).
-__device__ void restrict_level(const float* __restrict__ f, float* __restrict__ c, int nf, int nc) {
-  const float ratio = float(nf - 1) / float(nc - 1);
+template <typename T>
+__device__ void restrict_level(const T* __restrict__ f, T* __restrict__ c, int nf, int nc) {
+  const T ratio = T(nf - 1) / T(nc - 1);
   const int m = nc - 2;
   for (int q = threadIdx.x; q < m * m * m; q += kMgThreads) {
     const int i = q % m + 1, j = (q / m) % m + 1, k = q / (m * m) + 1;
-    float w[3][3];
+    T w[3][3];
     int ctr[3];
     const int idx[3] = {k, j, i};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      const float x = float(idx[a]) * ratio;
-      ctr[a] = int(x + 0.5f);
-      w[a][0] = 0.5f * (float(ctr[a]) + 0.5f - x);
-      w[a][2] = 0.5f - w[a][0];
-      w[a][1] = 0.5f;
+      const T x = T(idx[a]) * ratio;
+      ctr[a] = int(x + T(0.5));
+      w[a][0] = T(0.5) * (T(ctr[a]) + T(0.5) - x);
+      w[a][2] = T(0.5) - w[a][0];
+      w[a][1] = T(0.5);
     }
-    float s = 0.0f;
+    T s = T(0);
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -215,23 +218,24 @@ __device__ void restrict_level(const float* __restrict__ f, float* __restrict__ 
 
 // Trilinear prolongation added to the fine interior; terms in the order 000,001,010,100,011,110,101,111
 // of (z,y,x) (CPU_PoissonSolver_MG.cpp:635-684).
-__device__ void prolongate_add(const float* __restrict__ c, float* __restrict__ f, int nc, int nf) {
-  const float ratio = float(nc - 1) / float(nf - 1);
+template <typename T>
+__device__ void prolongate_add(const T* __restrict__ c, T* __restrict__ f, int nc, int nf) {
+  const T ratio = T(nc - 1) / T(nf - 1);
   const int m = nf - 2;
   const int order[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {0, 1, 1}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
   for (int q = threadIdx.x; q < m * m * m; q += kMgThreads) {
     const int i = q % m + 1, j = (q / m) % m + 1, k = q / (m * m) + 1;
-    float w[3][2];
+    T w[3][2];
     int lo[3];
     const int idx[3] = {k, j, i};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      const float x = float(idx[a]) * ratio;
+      const T x = T(idx[a]) * ratio;
       lo[a] = int(x);
-      w[a][0] = float(lo[a] + 1) - x;
-      w[a][1] = 1.0f - w[a][0];
+      w[a][0] = T(lo[a] + 1) - x;
+      w[a][1] = T(1) - w[a][0];
     }
-    float s = 0.0f;
+    T s = T(0);
 #pragma unroll
     for (int t = 0; t < 8; t++) {
       const int a = order[t][0], b = order[t][1], d = order[t][2];
@@ -243,31 +247,32 @@ __device__ void prolongate_add(const float* __restrict__ c, float* __restrict__ 
   __syncthreads();
 }
 
-// One patch: interpolation, V-cycles until Error <= tol or max_iter, window and optional gravity.
-template <int GS>
-__device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho, const float* __restrict__ cpot,
-                         const float* __restrict__ guess, float* __restrict__ out, float* __restrict__ flu,
-                         int* iters, int* status, const SolveParams& P) {
+// One patch: interpolation, V-cycles until Error <= tol or max_iter, window and optional gravity.  PT is the
+// parameter type, SolveParams (FP32) or SolveParams64 (FP64); T its element type.
+template <int GS, typename PT, typename T = typename PT::real>
+__device__ void mg_patch(T* __restrict__ lds, const T* __restrict__ rho, const T* __restrict__ cpot,
+                         const T* __restrict__ guess, T* __restrict__ out, T* __restrict__ flu, int* iters,
+                         int* status, const PT& P) {
   using M = MgGeo<GS>;
   constexpr int N = M::N, R = M::R, C = M::C, U = M::U, NLV = M::NLV;
-  float* sol0 = lds + M::off_sol(0);
-  float* red = lds + M::off_red();
+  T* sol0 = lds + M::off_sol(0);
+  T* red = lds + M::off_red();
 
   // interpolation into Sol0 (natural layout) and Def boundaries = 0
-  for (int t = threadIdx.x; t < M::off_red(); t += kMgThreads) lds[t] = 0.0f;
+  for (int t = threadIdx.x; t < M::off_red(); t += kMgThreads) lds[t] = T(0);
   __syncthreads();
   {
     constexpr int NC = C - 2;
     for (int q = threadIdx.x; q < NC * NC * NC; q += kMgThreads) {
       const int i = q % NC + 1, j = (q / NC) % NC + 1, k = q / (NC * NC) + 1;
-      float v[3][3][3];
+      T v[3][3][3];
 #pragma unroll
       for (int dk = 0; dk < 3; dk++)
 #pragma unroll
         for (int dj = 0; dj < 3; dj++)
 #pragma unroll
           for (int di = 0; di < 3; di++) v[dk][dj][di] = cpot[((k + dk - 1) * C + (j + dj - 1)) * C + (i + di - 1)];
-      float ch[2][2][2];
+      T ch[2][2][2];
       interp_children(v, P.int_scheme, ch);
 #pragma unroll
       for (int cz = 0; cz < 2; cz++)
@@ -283,20 +288,21 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
   __syncthreads();
 
   // level-0 right-hand side in registers; initial-guess modes
-  float rhs0[2][M::PER_THREAD];
+  T rhs0[2][M::PER_THREAD];
 #pragma unroll
   for (int c = 0; c < 2; c++)
 #pragma unroll
     for (int s = 0; s < M::PER_THREAD; s++) {
       const int q = threadIdx.x + kMgThreads * s;
-      rhs0[c][s] = 0.0f;
+      rhs0[c][s] = T(0);
       if (q < M::CELLS0) {
         int x, y, z;
         cell0<GS>(q, c, x, y, z);
         const int gi = ((z - 1) * R + (y - 1)) * R + (x - 1);
         rhs0[c][s] = P.poi_coeff * rho[gi];  // CPU_PoissonSolver_MG.cpp:200-207
-        if (P.init_mode == 1) sol0[nat(N, x, y, z)] = 0.0f;
+        if (P.init_mode == 1) sol0[nat(N, x, y, z)] = T(0);
         if (P.init_mode == 2) sol0[nat(N, x, y, z)] = guess[gi];
+        if (P.init_mode == 3) sol0[nat(N, x, y, z)] = guess[nat(N, x, y, z)];   // whole-cube guess: its interior
       }
     }
   __syncthreads();
@@ -305,8 +311,8 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
   // operand order, Const = Poi_Coeff dh^2, over the interior) exceeds guess_tol: start from the zero guess,
   // as the SOR kernel does, and report it in the status.
   bool fell_back = false;
-  if (P.init_mode == 2 && P.guess_tol > 0.0f) {
-    float num = 0.0f, den = 0.0f;
+  if (P.init_mode >= 2 && P.guess_tol > T(0)) {
+    T num = T(0), den = T(0);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -315,11 +321,11 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
         if (q < M::CELLS0) {
           int x, y, z;
           cell0<GS>(q, c, x, y, z);
-          const float* p = sol0 + nat(N, x, y, z);
-          const float r = p[N * N] + p[-N * N] + p[N] + p[-N] + p[1] + p[-1] - 6.0f * p[0] -
-                          P.cst * rho[((z - 1) * R + (y - 1)) * R + (x - 1)];
-          num += fabsf(r);
-          den += fabsf(p[0]);
+          const T* p = sol0 + nat(N, x, y, z);
+          const T r = p[N * N] + p[-N * N] + p[N] + p[-N] + p[1] + p[-1] - T(6) * p[0] -
+                      P.cst * rho[((z - 1) * R + (y - 1)) * R + (x - 1)];
+          num += fabs(r);
+          den += fabs(p[0]);
         }
       }
     block_sum2(num, den, red);
@@ -333,26 +339,26 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
           if (q < M::CELLS0) {
             int x, y, z;
             cell0<GS>(q, c, x, y, z);
-            sol0[nat(N, x, y, z)] = 0.0f;
+            sol0[nat(N, x, y, z)] = T(0);
           }
         }
     }
     __syncthreads();
   }
 
-  float dh[4];
+  T dh[4];
 #pragma unroll
-  for (int l = 0; l < 4; l++) dh[l] = l == 0 ? P.dh : P.dh * float(N - 1) / float(l < NLV ? M::n(l) - 1 : 1);
+  for (int l = 0; l < 4; l++) dh[l] = l == 0 ? P.dh : P.dh * T(N - 1) / T(l < NLV ? M::n(l) - 1 : 1);
 
   int iter = 0;
-  float error = FLT_MAX;
+  T error = T(FLT_MAX);
   while (iter < P.mg_max_iter && error > P.mg_tol) {
     // down: pre-smooth, defect, restrict, zero the next correction
 #pragma unroll
     for (int l = 0; l < NLV - 1; l++) {
       const int n = M::n(l);
-      float* sol = lds + M::off_sol(l);
-      float* def = lds + M::off_def(l);
+      T* sol = lds + M::off_sol(l);
+      T* def = lds + M::off_def(l);
       if (l == 0) {
         for (int s = 0; s < P.mg_npre; s++) smooth_level0<GS>(sol, rhs0, dh[0] * dh[0]);
         defect_level0<GS>(sol, def, rhs0, dh[0], false, red);
@@ -361,8 +367,8 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
         defect_coarse(sol, lds + M::off_rhs(l), def, dh[l], n);
       }
       restrict_level(def, lds + M::off_rhs(l + 1), n, M::n(l + 1));
-      float* nsol = lds + M::off_sol(l + 1);
-      for (int t = threadIdx.x; t < M::n3(l + 1); t += kMgThreads) nsol[t] = 0.0f;
+      T* nsol = lds + M::off_sol(l + 1);
+      for (int t = threadIdx.x; t < M::n3(l + 1); t += kMgThreads) nsol[t] = T(0);
       __syncthreads();
     }
     // bottom
@@ -388,7 +394,7 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
   // status: converged / max_iter / non-finite
   if (threadIdx.x == 0) {
     int st = error <= P.mg_tol ? kStatusConverged : kStatusMaxIter;
-    if (!(error == error) || error == INFINITY) st |= kStatusNonFinite;
+    if (!(error == error) || error == T(INFINITY)) st |= kStatusNonFinite;
     if (fell_back) st |= kStatusFallback;
     if (iters) *iters = iter;
     if (status) *status = st;
@@ -405,20 +411,20 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
       const int i = t % kPatchSize, j = (t / kPatchSize) % kPatchSize, k = t / (kPatchSize * kPatchSize);
       const int x = i + GS, y = j + GS, z = k + GS;
       auto Q = [&](int dz, int dy, int dx) { return sol0[nat(N, x + dx, y + dy, z + dz)]; };
-      float ax, ay, az;
+      T ax, ay, az;
       if (P.p5) {
-        ax = P.gra_const * (-Q(0, 0, 2) + Q(0, 0, -2) + 8.0f * Q(0, 0, 1) - 8.0f * Q(0, 0, -1));
-        ay = P.gra_const * (-Q(0, 2, 0) + Q(0, -2, 0) + 8.0f * Q(0, 1, 0) - 8.0f * Q(0, -1, 0));
-        az = P.gra_const * (-Q(2, 0, 0) + Q(-2, 0, 0) + 8.0f * Q(1, 0, 0) - 8.0f * Q(-1, 0, 0));
+        ax = P.gra_const * (-Q(0, 0, 2) + Q(0, 0, -2) + T(8) * Q(0, 0, 1) - T(8) * Q(0, 0, -1));
+        ay = P.gra_const * (-Q(0, 2, 0) + Q(0, -2, 0) + T(8) * Q(0, 1, 0) - T(8) * Q(0, -1, 0));
+        az = P.gra_const * (-Q(2, 0, 0) + Q(-2, 0, 0) + T(8) * Q(1, 0, 0) - T(8) * Q(-1, 0, 0));
       } else {
         ax = P.gra_const * (Q(0, 0, 1) - Q(0, 0, -1));
         ay = P.gra_const * (Q(0, 1, 0) - Q(0, -1, 0));
         az = P.gra_const * (Q(1, 0, 0) - Q(-1, 0, 0));
       }
-      const float rho_f = flu[t];
-      float px = flu[kPatchCells + t], py = flu[2 * kPatchCells + t], pz = flu[3 * kPatchCells + t];
-      const float temp = 0.5f / rho_f;
-      const float eint = flu[4 * kPatchCells + t] - temp * (px * px + py * py + pz * pz);
+      const T rho_f = flu[t];
+      T px = flu[kPatchCells + t], py = flu[2 * kPatchCells + t], pz = flu[3 * kPatchCells + t];
+      const T temp = T(0.5) / rho_f;
+      const T eint = flu[4 * kPatchCells + t] - temp * (px * px + py * py + pz * pz);
       px = px - rho_f * ax;
       py = py - rho_f * ay;
       pz = pz - rho_f * az;
@@ -431,15 +437,19 @@ __device__ void mg_patch(float* __restrict__ lds, const float* __restrict__ rho,
   __syncthreads();
 }
 
-template <int GS>
-__global__ void __launch_bounds__(kMgThreads) mg_uniform_kernel(UniformBatch b, SolveParams P) {
-  extern __shared__ float lds[];
+template <int GS, typename PT>
+__global__ void __launch_bounds__(kMgThreads) mg_uniform_kernel(UniformBatchT<typename PT::real> b, PT P) {
+  using T = typename PT::real;
+  extern __shared__ __align__(16) unsigned char mg_smem[];
+  T* lds = reinterpret_cast<T*>(mg_smem);
   using M = MgGeo<GS>;
   constexpr int R3 = M::R * M::R * M::R, C3 = M::C * M::C * M::C;
   const int p = blockIdx.x;
   if (p >= b.npatch) return;
   const int G = gra_nxt(P.gra_ghost);
-  mg_patch<GS>(lds, b.rho + (size_t)p * R3, b.cpot + (size_t)p * C3, b.guess ? b.guess + (size_t)p * R3 : nullptr,
+  constexpr int N3 = M::N * M::N * M::N;
+  mg_patch<GS>(lds, b.rho + (size_t)p * R3, b.cpot + (size_t)p * C3,
+               b.guess ? b.guess + (size_t)p * (P.init_mode == 3 ? N3 : R3) : nullptr,
                b.pot_out + (size_t)p * G * G * G, b.flu ? b.flu + (size_t)p * kNComp * kPatchCells : nullptr,
                b.iters ? b.iters + p : nullptr, b.status ? b.status + p : nullptr, P);
 }
@@ -455,10 +465,13 @@ size_t lds_bytes_mg(int gs) {
   return 0;
 }
 
-template <int GS>
-static hipError_t launch_mg(const UniformBatch& b, const SolveParams& p, hipStream_t s) {
-  const size_t lds = lds_bytes_mg(GS);
-  hipLaunchKernelGGL((mg_uniform_kernel<GS>), dim3(b.npatch), dim3(kMgThreads), lds, s, b, p);
+template <int GS, typename PT>
+static hipError_t launch_mg(const UniformBatchT<typename PT::real>& b, const PT& p, hipStream_t s) {
+  const size_t lds = lds_bytes_mg(GS) / sizeof(float) * sizeof(typename PT::real);
+  if (lds > 64 * 1024)   // the FP64 levels of the larger cubes exceed the default dynamic LDS limit
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mg_uniform_kernel<GS, PT>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((mg_uniform_kernel<GS, PT>), dim3(b.npatch), dim3(kMgThreads), lds, s, b, p);
   return hipGetLastError();
 }
 
@@ -539,6 +552,17 @@ hipError_t launch_mg_persistent(const MixedBatch& b, const ParamsByGs& p, int gr
 }
 
 hipError_t launch_mg_uniform(const UniformBatch& b, const SolveParams& p, hipStream_t s) {
+  switch (b.gs) {
+    case 1: return launch_mg<1>(b, p, s);
+    case 2: return launch_mg<2>(b, p, s);
+    case 3: return launch_mg<3>(b, p, s);
+    case 4: return launch_mg<4>(b, p, s);
+    case 5: return launch_mg<5>(b, p, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_mg_uniform64(const UniformBatch64& b, const SolveParams64& p, hipStream_t s) {
   switch (b.gs) {
     case 1: return launch_mg<1>(b, p, s);
     case 2: return launch_mg<2>(b, p, s);

@@ -155,6 +155,8 @@ int check_mixed_args(pgamd_context* c, pgamd_plan* pl, const float* rho, const f
     return set_error(PGAMD_ERR_ARG, "null device array");
   if (a->poisson && o.init_mode == PGAMD_INIT_GUESS && !guess)
     return set_error(PGAMD_ERR_ARG, "init_mode GUESS needs a guess array");
+  if (o.init_mode == PGAMD_INIT_GUESS_CUBE)
+    return set_error(PGAMD_ERR_ARG, "mixed batches take guesses packed like the density (init_mode GUESS)");
   return PGAMD_OK;
 }
 
@@ -353,8 +355,8 @@ int pgamd_graph_capture_device(pgamd_context* c, const float* rho, const float* 
   if (npatch <= 0) return set_error(PGAMD_ERR_ARG, "a graph needs at least one patch");
   if ((a->poisson && (!rho || !cpot)) || !pot_out || (a->graacc && !flu))
     return set_error(PGAMD_ERR_ARG, "null device array");
-  if (a->poisson && ctx_options(c).init_mode == PGAMD_INIT_GUESS && !guess)
-    return set_error(PGAMD_ERR_ARG, "init_mode GUESS needs a guess array");
+  if (a->poisson && ctx_options(c).init_mode >= PGAMD_INIT_GUESS && !guess)
+    return set_error(PGAMD_ERR_ARG, "init_mode GUESS / GUESS_CUBE needs a guess array");
   DeviceArgs d{rho, cpot, guess, pot_out, flu, npatch, a, iters, status};
   return capture(
       c, out,
@@ -406,7 +408,7 @@ int pgamd_autotune(pgamd_context* c, int gs, const float* rho, const float* cpot
   o.pot_scheme = PGAMD_SOR;
   o.fuse_gravity = 0;
   // the candidates are timed without a guess array: a GUESS start is timed as the zero start (same sweeps)
-  if (o.init_mode == PGAMD_INIT_GUESS) o.init_mode = PGAMD_INIT_ZERO;
+  if (o.init_mode >= PGAMD_INIT_GUESS) o.init_mode = PGAMD_INIT_ZERO;
   o.guess_fallback_tol = 0.0f;
   pgamd_solver_args at = *a;
   at.graacc = 0;

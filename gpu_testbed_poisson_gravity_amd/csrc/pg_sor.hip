@@ -152,6 +152,33 @@ __device__ __forceinline__ void interp_to_lds(const float* __restrict__ cp, floa
   }
 }
 
+// The Dirichlet layer of a patch from a caller's whole-cube guess [N^3] into its LDS cube: every load of the
+// lane is issued before any value is stored, one memory round trip for the phase.
+template <int GS, int SEG>
+__device__ __forceinline__ void shell_to_lds(const float* __restrict__ g, float* __restrict__ lds, int lane) {
+  using S = SorGeo<GS>;
+  constexpr int N = S::N, HP = S::HP;
+  constexpr int NSHELL = 2 * N * N + 4 * (N - 1) * (N - 2);
+  constexpr int PER = (NSHELL + SEG - 1) / SEG;
+  float v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int q = lane + k * SEG;
+    int x, y, z;
+    shell_cell<N>(q < NSHELL ? q : 0, x, y, z);
+    v[k] = g[(z * N + y) * N + x];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int q = lane + k * SEG;
+    if (q < NSHELL) {
+      int x, y, z;
+      shell_cell<N>(q, x, y, z);
+      lds[cube_index(N, HP, x, y, z)] = v[k];
+    }
+  }
+}
+
 // ---- per-lane state and sweeps -----------------------------------------------------------------------
 
 template <int GS>
@@ -331,7 +358,7 @@ __device__ void gravity_from_lds(const float* __restrict__ lds, float* __restric
 
 // Solve up to PPW patches of ghost size GS with one wavefront (blockDim.x == 64).
 template <int GS, int PPW>
-__device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, const SolveParams& P) {
+__device__ __forceinline__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, const SolveParams& P) {
   using S = SorGeo<GS>;
   constexpr int N = S::N, R = S::R, HP = S::HP, MX = S::MX, LP = S::LP;
   constexpr int SEG = 64 / PPW;
@@ -349,9 +376,11 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
 
   // A. coarse -> fine interpolation of the whole cube (initial guess and Dirichlet layer); a zero start or a
   // caller's guess needs only the Dirichlet layer (a rejected guess restarts from zero, not from the
-  // interpolation)
+  // interpolation), and a whole-cube guess brings its own Dirichlet layer
   if (lp.valid) {
-    if (P.init_mode != 0)
+    if (P.init_mode == 3)
+      shell_to_lds<GS, SEG>(lp.guess, lds, pl);
+    else if (P.init_mode != 0)
       interp_to_lds<GS, true>(lp.cpot, lds, pl, SEG, P.int_scheme);
     else
       interp_to_lds<GS, false>(lp.cpot, lds, pl, SEG, P.int_scheme);
@@ -380,6 +409,14 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
 #pragma unroll
         for (int zi = 0; zi < R; zi++)
           g2[yy][zi] = *reinterpret_cast<const float2*>(lp.guess + (zi * R + 2 * yb + yy) * R + 2 * m);
+    } else if (P.init_mode == 3) {   // whole-cube guess: the owned cells sit at cube x = 2m+1, 2m+2 (odd start)
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int zi = 0; zi < R; zi++) {
+          const float* gp = lp.guess + ((zi + 1) * N + 2 * yb + yy + 1) * N + 2 * m + 1;
+          g2[yy][zi] = make_float2(gp[0], gp[1]);
+        }
     }
   }
 #pragma unroll
@@ -412,7 +449,7 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
   // B'. reject a caller guess whose relative residual exceeds guess_tol: restart from the zero guess
   float res[2 * R];
   bool fell_back = false;
-  if (P.init_mode == 2 && P.guess_tol > 0.0f) {
+  if (P.init_mode >= 2 && P.guess_tol > 0.0f) {
     float num = half_sweep<GS, 1, 1, true>(L, lds, lb, 0.0f, res);
     num += half_sweep<GS, 0, 1, true>(L, lds, lb, 0.0f, res);
     float den = 0.0f;
@@ -531,15 +568,15 @@ __device__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, cons
 // ---- kernels -----------------------------------------------------------------------------------------
 
 template <int GS>
-__device__ __forceinline__ LanePatch uniform_patch(const UniformBatch& b, int patch) {
+__device__ __forceinline__ LanePatch uniform_patch(const UniformBatch& b, int patch, int init_mode) {
   using S = SorGeo<GS>;
-  constexpr int R3 = S::R * S::R * S::R, C3 = S::C * S::C * S::C;
+  constexpr int R3 = S::R * S::R * S::R, C3 = S::C * S::C * S::C, N3 = S::N * S::N * S::N;
   LanePatch lp;
   lp.valid = patch < b.npatch;
   const int p = lp.valid ? patch : 0;
   lp.rho = b.rho + (size_t)p * R3;
   lp.cpot = b.cpot + (size_t)p * C3;
-  lp.guess = b.guess ? b.guess + (size_t)p * R3 : nullptr;
+  lp.guess = b.guess ? b.guess + (size_t)p * (init_mode == 3 ? N3 : R3) : nullptr;
   lp.out = nullptr;  // set by the caller (depends on the window size)
   lp.flu = b.flu ? b.flu + (size_t)p * kNComp * kPatchCells : nullptr;
   lp.iters = b.iters ? b.iters + p : nullptr;
@@ -553,7 +590,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
   constexpr int SEG = 64 / PPW;
   const int slot = threadIdx.x / SEG;
   const int patch = blockIdx.x * PPW + slot;
-  LanePatch lp = uniform_patch<GS>(b, patch);
+  LanePatch lp = uniform_patch<GS>(b, patch, P.init_mode);
   const int G = gra_nxt(P.gra_ghost);
   lp.out = b.pot_out + (size_t)(lp.valid ? patch : 0) * G * G * G;
   sor_task<GS, PPW>(lds, lp, P);

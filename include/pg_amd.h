@@ -37,11 +37,16 @@ extern "C" {
 #define PGAMD_INT_CQUAD 4
 #define PGAMD_INT_QUAD 5
 
-/* Initial guess of the interior (the Dirichlet layer always comes from the interpolated coarse potential).
- * The reference always starts from the interpolated cube (CPU_PoissonSolver_SOR.cpp:65). */
+/* Initial guess of the interior.  The reference always starts from the interpolated cube
+ * (CPU_PoissonSolver_SOR.cpp:65).  INTERP, ZERO, GUESS: the Dirichlet layer comes from the interpolated coarse
+ * potential; GUESS takes a caller's interior [npatch][R^3].  GUESS_CUBE takes a caller's whole solve cube
+ * [npatch][N^3], N = R + 2: its interior is the start and SOR also takes its Dirichlet layer, which must then
+ * be the interpolated coarse potential for the reference problem (pgamd_cnn_warm_start_cube writes exactly
+ * that); MG interpolates the layer itself.  GUESS_CUBE is for uniform batches (pgamd_solve_device, graphs). */
 #define PGAMD_INIT_INTERP 0
 #define PGAMD_INIT_ZERO 1
 #define PGAMD_INIT_GUESS 2
+#define PGAMD_INIT_GUESS_CUBE 3
 
 /* SOR stopping rule.  REFERENCE: stop once the L1 residual of an iteration grows (after Min_Iter),
  * CPU_PoissonSolver_SOR.cpp:273-283.  The device sums |R| as a wavefront tree, the reference CPU
@@ -72,7 +77,7 @@ typedef struct {
   int check_every;      /* >= 1                                                    */
   float rel_tol;        /* RELRES tolerance                                        */
   int fuse_gravity;     /* 1: gravity update inside the solve kernel, phi still in LDS */
-  float guess_fallback_tol; /* init GUESS, SOR and MG: use the zero guess instead when the guess's relative
+  float guess_fallback_tol; /* init GUESS(_CUBE), SOR and MG: use the zero guess instead when the guess's relative
                                residual sum|R| / sum|phi| exceeds this (0: never) */
   int mixed_persistent; /* mixed batches (pgamd_solve_mixed_device, graphs): 0 = one launch per ghost size and
                            scheme, one task per workgroup; 1 = one persistent work-queue launch per scheme.
@@ -153,6 +158,28 @@ int pgamd_solve_device(pgamd_context* ctx, const float* d_rho, const float* d_po
                        float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* args, int* d_iters,
                        int* d_status, void* stream);
 
+/* ---- FP64: the reference's FLOAT8 build (Makefile:22, Header/Macro.h:35), where every array and constant is
+ * double.  Same arguments and options as pgamd_solve_device with double arrays and double constants
+ * (Const = Poi_Coeff dh^2, Omega/6 and Gra_Const formed in double).  Uniform batches, SOR or MG, init modes
+ * INTERP / ZERO / GUESS / GUESS_CUBE, both stopping rules, fused or separate gravity.  The FP64 kernels keep the
+ * reference's operation order: at equal iteration counts the results are bitwise those of the reference's FP64
+ * CPU solver. */
+typedef struct {
+  double dt, dh;
+  int sor_min_iter, sor_max_iter;
+  double sor_omega;
+  int mg_max_iter, mg_npre_smooth, mg_npost_smooth;
+  double mg_tolerated_error;
+  double poi_coeff;
+  int int_scheme;
+  int p5_gradient;
+  int poisson, graacc;
+} pgamd_solver_args_f64;
+
+int pgamd_solve_device_f64(pgamd_context* ctx, const double* d_rho, const double* d_pot_in, const double* d_guess,
+                           double* d_pot_out, double* d_flu, int npatch, const pgamd_solver_args_f64* args,
+                           int* d_iters, int* d_status, void* stream);
+
 /* Autotuner: times every supported layout of one ghost size on the caller's device batch (SOR with the
  * context's options, gravity off; repeats >= 1 timed solves per candidate after one warm-up), keeps the
  * fastest in the context and returns it.  d_pot_out is overwritten.  The layout table can be saved and
@@ -228,6 +255,12 @@ int pgamd_cnn_train_step(pgamd_cnn* net, const uint16_t* d_x0, const float* d_ta
 /* Guess of the interior ([n][R^3]) for a batch at the context's ghost size: prepare, forward, assemble. */
 int pgamd_cnn_warm_start(pgamd_context* ctx, pgamd_cnn* net, const float* d_rho, const float* d_pot_in, int npatch,
                          const pgamd_solver_args* args, float* d_guess, void* stream);
+/* The same guess as the whole solve cube ([n][N^3], N = R + 2) for init_mode PGAMD_INIT_GUESS_CUBE: the interior is
+ * the network's guess (bitwise that of pgamd_cnn_warm_start) and the Dirichlet layer the interpolated coarse
+ * potential (bitwise that of the solver's own interpolation), so the solve reads its boundary instead of
+ * interpolating it. */
+int pgamd_cnn_warm_start_cube(pgamd_context* ctx, pgamd_cnn* net, const float* d_rho, const float* d_pot_in,
+                              int npatch, const pgamd_solver_args* args, float* d_guess_cube, void* stream);
 /* Warm start and solve of a uniform batch in one call, pipelined over nchunk (1..16) slices: the network
  * of slice i+1 runs on the context's second stream while slice i is solved on `stream`.  The context must
  * have init_mode PGAMD_INIT_GUESS (guess_fallback_tol applies); d_guess ([n][R^3]) is workspace and

@@ -6,8 +6,9 @@ TEST INFRASTRUCTURE ONLY: used by tests/ to check the HIP kernels.  The network 
 counterpart; its definition is the one stated in include/pg_amd.h (pgamd_cnn_*):
 
     fields on the 18^3 frame (the solve cube of ghost size 5; smaller cubes centred, zeros around):
-      q0 = interp / s,  q1 = 64 * r / s (solve interior only), s = max |interp|, r the SOR residual of the
-      interpolated cube (the right-hand side of the equation the correction satisfies)
+      q0 = interp / s,  q1 = 64 * r / s, both on the solve interior only (zero elsewhere), s = max |interp|
+      over the solve cube, r the SOR residual of the interpolated cube (the right-hand side of the equation
+      the correction satisfies; the boundary enters through r)
     x0 [n,10,10,10,16]: space-to-depth of the 20^3 zero-extended frame, fine index f = 2c - 1 + d
       (c = 0..9, d = 0..1), channel ((dz*2+dy)*2+dx)*2 + q
     y1 = relu(conv(x0) + b1)                                  (valid 3^3 conv, 10^3 -> 8^3)
@@ -62,15 +63,16 @@ RES_SCALE = np.float32(64.0)
 def frame18(cube, rho, gs, cst, scale):
     """(q0, q1) on the 18^3 frame for one patch: cube [N^3] interpolated potential, rho [R^3].
 
-    q0 = cube / s; q1 = 64 * r / s on the solve interior, r the SOR residual of the interpolated cube in
-    the reference operand order ((((((kp+km)+jp)+jm)+ip)+im) - 6 phi) - Const*rho, all in FP32."""
+    On the solve interior (zero elsewhere): q0 = cube / s; q1 = 64 * r / s, r the SOR residual of the
+    interpolated cube in the reference operand order ((((((kp+km)+jp)+jm)+ip)+im) - 6 phi) - Const*rho, all in
+    FP32."""
     N, R, off = 8 + 2 * gs, 6 + 2 * gs, 5 - gs
     c = np.asarray(cube, np.float32)
     q0 = np.zeros((18, 18, 18), np.float32)
     q1 = np.zeros((18, 18, 18), np.float32)
     inv = np.float32(1.0) / np.float32(scale)
-    q0[off:off + N, off:off + N, off:off + N] = c * inv
     i = slice(1, N - 1)
+    q0[off + 1:off + 1 + R, off + 1:off + 1 + R, off + 1:off + 1 + R] = c[i, i, i] * inv
     r = (((((c[2:, i, i] + c[:-2, i, i]) + c[i, 2:, i]) + c[i, :-2, i]) + c[i, i, 2:]) + c[i, i, :-2])
     r = (r - np.float32(6.0) * c[i, i, i]) - np.float32(cst) * np.asarray(rho, np.float32)
     q1[off + 1:off + 1 + R, off + 1:off + 1 + R, off + 1:off + 1 + R] = (r * inv) * RES_SCALE

@@ -120,10 +120,13 @@ def test_frame_centres_smaller_cubes():
     cube = (ax[:, None, None] ** 2 + ax[None, :, None] ** 2 + ax[None, None, :] ** 2).astype(np.float32)
     rho = np.ones((8, 8, 8), np.float32)
     q0, q1 = K.frame18(cube, rho, gs=1, cst=2.0, scale=4.0)
-    assert q0[4, 4, 4] == cube[0, 0, 0] / 4 and q0[5, 4, 4] == cube[1, 0, 0] / 4
-    assert q0[3].max() == 0 and q0[14].max() == 0                # gs 1: the 10^3 cube sits at 4..13
+    # gs 1: the 10^3 cube sits at 4..13, its 8^3 interior at 5..12; both fields live on the interior only
+    assert q0[5, 5, 5] == cube[1, 1, 1] / 4 and q0[6, 5, 5] == cube[2, 1, 1] / 4
+    assert np.array_equal(q0[5:13, 5:13, 5:13], cube[1:9, 1:9, 1:9] / np.float32(4))
     assert np.all(q1[5:13, 5:13, 5:13] == 64.0 * (6.0 - 2.0) / 4.0)
-    assert q1[4].max() == 0 and q1[13].max() == 0 and q1[:, 4].max() == 0   # Dirichlet layer and outside
+    for q in (q0, q1):   # the Dirichlet layer (4 and 13) and outside are zero
+        assert q[4].max() == 0 and q[13].max() == 0 and q[:, 4].max() == 0 and q[:, :, 13].max() == 0
+        assert q[:4].max() == 0 and q[14:].max() == 0
 
 
 def test_bf16_round_ties_to_even():

@@ -278,6 +278,48 @@ def test_warm_start_guess_assembly(gs):
     assert np.array_equal(d_guess.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("gs", [2, 5])
+@pytest.mark.parametrize("scheme", [pg.SOR, pg.MG])
+def test_whole_cube_guess_matches_interior_guess(gs, scheme):
+    """pgamd_cnn_warm_start_cube: the interior is bitwise the interior guess and the Dirichlet layer bitwise the
+    interpolated coarse potential; a solve from it (init GUESS_CUBE, SOR reading its Dirichlet layer from the
+    guess) is bitwise the solve from the interior guess, with and without the guess check."""
+    torch = _torch()
+    n = 12
+    rho, cpot, flu, _ = I.plummer(n, gs, seed=60 + gs)
+    args = pg.solver_args(dh=1.0, poi_coeff=POI, sor_min_iter=25, sor_max_iter=25, sor_omega=1.7, mg_max_iter=3,
+                          mg_tol=0.0, mg_npre=2, mg_npost=2)
+    g = pg.geometry(gs)
+    R, N = g["R"], g["N"]
+    with pg.Context(0) as c:
+        c.set_options(pot_ghost_size=gs, pot_scheme=scheme)
+        net = pg.CNN(c, seed=2)
+        net.set_params(_random_params(7))
+        try:
+            d_rho, d_cpot = _dev(rho), _dev(cpot)
+            d_int = torch.empty((n, R, R, R), device="cuda")
+            d_cube = torch.full((n, N, N, N), float("nan"), device="cuda")
+            print(f"[case] whole-cube guess gs={gs} scheme={scheme}", flush=True)
+            net.warm_start(d_rho, d_cpot, n, args, d_int)
+            net.warm_start_cube(d_rho, d_cpot, n, args, d_cube)
+            torch.cuda.synchronize()
+            cube_g, int_g = d_cube.cpu().numpy(), d_int.cpu().numpy()
+            assert np.array_equal(cube_g[:, 1:-1, 1:-1, 1:-1], int_g)
+            interp = O.interp(cpot, gs, pg.INT_CQUAD, dtype=np.float32)
+            face = np.ones((N, N, N), bool)
+            face[1:-1, 1:-1, 1:-1] = False
+            assert np.array_equal(cube_g[:, face], interp[:, face])
+            for tol in (0.0, 1e-2):
+                c.set_options(init_mode=pg.INIT_GUESS, guess_fallback_tol=tol)
+                a = _solve_dev(c, rho, cpot, flu, args, guess=d_int)
+                c.set_options(init_mode=pg.INIT_GUESS_CUBE)
+                b = _solve_dev(c, rho, cpot, flu, args, guess=d_cube)
+                for u, v in zip(a, b):
+                    assert np.array_equal(u, v)
+        finally:
+            net.close()
+
+
 def test_guess_fallback_to_zero_guess():
     """With guess_fallback_tol > 0 the SOR kernel rejects a guess whose relative residual exceeds the
     tolerance, restarts from the zero guess and sets status bit 8; an accepted guess is used as given."""
