@@ -137,6 +137,7 @@ SIGNATURES = {
     "pgamd_cnn_prepare": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(SolverArgs),
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
     "pgamd_cnn_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "pgamd_cnn_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "pgamd_cnn_loss_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _FP, C.c_void_p,
                                       C.c_void_p]),
     "pgamd_cnn_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float, _FP,
@@ -495,6 +496,10 @@ class CNN:
 
     def forward(self, x0, count, out, stream=None):
         _check(load().pgamd_cnn_forward(self.h, _vp(x0), count, _vp(out), stream))
+
+    def forward_noalloc(self, x0, count, out, stream=None):
+        """pgamd_cnn_forward without allocation (at most 64 samples; activations in the library's fixed array)."""
+        _check(load().pgamd_cnn_forward_noalloc(self.h, _vp(x0), count, _vp(out), stream))
 
     def set_loss(self, res_weight: float = 0.0):
         """Loss = MSE + res_weight * mean squared residual the error leaves in the guess (0: plain MSE)."""

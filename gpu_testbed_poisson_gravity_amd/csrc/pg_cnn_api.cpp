@@ -171,6 +171,12 @@ float read_loss(pgamd_cnn* n, int count, hipStream_t s) {
 
 }  // namespace
 
+namespace pgamd {
+int cnn_device(const pgamd_cnn* n) { return n->device; }
+const uint16_t* cnn_forward_weights(const pgamd_cnn* n, int l) { return n->wb + (size_t)(l - 1) * kWF; }
+const float* cnn_bias(const pgamd_cnn* n, int l) { return n->p + ob(l); }
+}  // namespace pgamd
+
 extern "C" {
 
 int pgamd_cnn_num_params(void) { return kCnnParams; }

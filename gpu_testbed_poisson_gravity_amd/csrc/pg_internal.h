@@ -30,6 +30,11 @@ const PatchDesc* plan_device_desc(const pgamd_plan* pl);
 int plan_npatch(const pgamd_plan* pl);
 int plan_device(const pgamd_plan* pl);
 bool plan_has_ghost(const pgamd_plan* pl, int gs);
+// Network internals for the allocation-free forward (layer l = 1..6): the device, the BF16 forward-layout
+// weights and the FP32 biases.
+int cnn_device(const pgamd_cnn* n);
+const uint16_t* cnn_forward_weights(const pgamd_cnn* n, int l);
+const float* cnn_bias(const pgamd_cnn* n, int l);
 // Frees whatever the optional modules (graphs, mixed batches, CNN) attached to a context.
 void release_extras(pgamd_context* c);
 

@@ -240,6 +240,12 @@ int pgamd_cnn_set_params(pgamd_cnn* net, const float* host_params, int count);
 int pgamd_cnn_prepare(pgamd_context* ctx, const float* d_rho, const float* d_pot_in, int npatch, int ghost_size,
                       const pgamd_solver_args* args, uint16_t* d_x0, float* d_scale, void* stream);
 int pgamd_cnn_forward(pgamd_cnn* net, const uint16_t* d_x0, int count, float* d_out, void* stream);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same forward as pgamd_cnn_forward, bitwise, with its
+ * activations in a fixed device array of the library (10.24 MB per GPU, held until the process ends) instead
+ * of an allocated workspace: at most 64 samples per call (PGAMD_ERR_ARG beyond), no allocation, no host
+ * synchronisation.  Calls that overlap on one GPU share that array and race. */
+int pgamd_cnn_forward_noalloc(pgamd_cnn* net, const uint16_t* d_x0, int count, float* d_out, void* stream);
 /* Training loss: mean squared error of the output against d_target ([n][8^3][8]) plus res_weight times the mean
  * squared residual that error leaves in the guess, r = (sum of the six neighbours - 6 e) on the 16^3 frame
  * (e = output - target at frame cell 2o + p, zero outside), both means over the n*4096 outputs.  res_weight = 0
