@@ -98,6 +98,9 @@ def parse():
     ap.add_argument("--asyn-streams", default="1,2,4,8",
                     help="stream counts of the host-array pipeline leg (pgamd_asyn_poisson_gravity_solver with pinned "
                          "buffers, PCIe included; empty: skip)")
+    ap.add_argument("--level-box", type=int, default=12,
+                    help="refined box of M^3 coarse patches (8 M^3 fine patches) for the prepare/solve/scatter "
+                         "cycle of two AMR levels; 0 skips it")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size reference for "
                          "rocprof FETCH_SIZE / WRITE_SIZE calibration)")
@@ -310,6 +313,73 @@ def asyn_pipeline(a, pg, torch, ctx, rho, cpot, flu, args, kernel_ms):
             "note": "PCIe-inclusive rate of the host-array entry: never the headline value"}
 
 
+def level_cycle(a, pg, ctx, args):
+    """Two AMR levels through the host-side cycle a GAMER time step runs: pgamd_prepare_patch_groups gathers
+    every patch group of the refined box (density with ghost cells from the fine siblings, coarse potential from
+    the father's neighbourhood, fluid) into the pinned buffers, pgamd_asyn_poisson_gravity_solver solves them
+    (PCIe included), pgamd_scatter_patch_groups writes potential and fluid back.  A Plummer sphere (M = 1,
+    a = 6 cells) at the centre of the box; each stage timed on the host clock (median of the timed calls)."""
+    from gpu_testbed_poisson_gravity_amd import inputs as I
+
+    m, gs = a.level_box, a.gs
+    ncp = m + 2
+    F, L = 16 * m, 8 * ncp
+    c = 8.0 * ncp
+    f = 16.0 + np.arange(F) + 0.5
+    r2 = (f[:, None, None] - c) ** 2 + (f[None, :, None] - c) ** 2 + (f[None, None, :] - c) ** 2
+    fine = np.zeros((5, F, F, F), np.float32)
+    fine[0] = 3.0 / (4.0 * np.pi * 6.0 ** 3) * (1.0 + r2 / 36.0) ** -2.5
+    fine[4] = 1.0
+    del r2
+    X = 2.0 * np.arange(L) + 1.0
+    coarse = (-I.NEWTON_G / np.sqrt((X[:, None, None] - c) ** 2 + (X[None, :, None] - c) ** 2 +
+                                    (X[None, None, :] - c) ** 2 + 36.0)).astype(np.float32)
+    d = I.two_level_box(fine, coarse, (1, 1, 1), seed=a.seed)
+    del fine
+    ng = d["group_first"].size
+    n = 8 * ng
+    ctx.set_options(pot_ghost_size=gs, gra_ghost_size=1, pot_scheme=pg.SOR, init_mode=pg.INIT_INTERP,
+                    term_mode=pg.TERM_RELRES, check_every=a.check_every, rel_tol=1e-5, fuse_gravity=1,
+                    guess_fallback_tol=0.0)
+    h_rho, h_cpot, h_out, h_flu = ctx.mem_allocate(ng)
+    fine_pot = np.zeros((n, 8, 8, 8), np.float32)
+    fine_flu = d["fine_flu"].copy()
+    nt = min(16, len(os.sched_getaffinity(0)))
+    ts = {"prepare": [], "solve": [], "scatter": []}
+    for k in range(a.warmup + max(3, a.steps // 4)):
+        t0 = time.perf_counter()
+        pg.prepare_patch_groups(gs, d["group_first"], d["group_father"], d["fine_flu"], d["fine_sib"],
+                                d["coarse_pot"], d["coarse_sib"], h_rho=h_rho, h_pot_in=h_cpot, h_flu=h_flu,
+                                nthreads=nt)
+        t1 = time.perf_counter()
+        ctx.asyn_poisson_gravity_solver(h_rho, h_cpot, h_out, h_flu, ng, args.dt, args.dh, args.sor_min_iter,
+                                        args.sor_max_iter, args.sor_omega, args.mg_max_iter, args.mg_npre_smooth,
+                                        args.mg_npost_smooth, args.mg_tolerated_error, args.poi_coeff,
+                                        args.int_scheme, False, True, True, gpu_nstream=4)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        pg.scatter_patch_groups(1, d["group_first"], h_out, h_flu, fine_pot=fine_pot, fine_flu=fine_flu, nthreads=nt)
+        t3 = time.perf_counter()
+        if k >= a.warmup:
+            ts["prepare"].append(t1 - t0)
+            ts["solve"].append(t2 - t1)
+            ts["scatter"].append(t3 - t2)
+    it, st = ctx.patch_stats(n)
+    ctx.mem_free()
+    ms = {k: float(np.median(v)) * 1e3 for k, v in ts.items()}
+    tot = sum(ms.values())
+    g = pg.geometry(gs)
+    moved = n * 4.0 * (g["R"] ** 3 + g["C"] ** 3 + 2 * 5 * 512)      # bytes written into the pinned buffers
+    out = {"fine_patches": n, "patch_groups": ng, "coarse_patches": int(ncp ** 3), "host_threads": nt,
+           "prepare_ms": ms["prepare"], "solve_ms": ms["solve"], "scatter_ms": ms["scatter"], "total_ms": tot,
+           "patch_solves_per_s": n / (tot * 1e-3), "prepare_GBps": moved / (ms["prepare"] * 1e-3) / 1e9,
+           "mean_iters": float(it.mean()), "converged": int(np.sum((st & pg.STATUS_CONVERGED) != 0)),
+           "note": "host-side AMR cycle (gather, PCIe-inclusive solve, scatter): never the headline value"}
+    log(f"[bench] level cycle ({n} fine patches, {nt} host threads): prepare {ms['prepare']:.2f} ms, solve "
+        f"{ms['solve']:.2f} ms, scatter {ms['scatter']:.2f} ms -> {out['patch_solves_per_s']:.4g} patch-solves/s")
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -402,6 +472,8 @@ def main():
                 "fallbacks": int(((st & pg.STATUS_FALLBACK) != 0).sum()), "iters": it}
 
     traffic = measured_traffic(a)
+    occ = ctx.layout_occupancy(a.gs)
+    log(f"[bench] SOR residency at ghost size {a.gs}: {occ[0]} workgroups, {occ[1]} patches per CU")
 
     def lds_roof(r, leg):
         alg = float(r["iters"].astype(np.float64).sum()) * R ** 3 * LDS_BYTES_PER_UPDATE
@@ -412,6 +484,7 @@ def main():
             leg, N ** 3 if a.guess == "cube" else R ** 3 + g["C"] ** 3)
         hbm = a.npatch * 4.0 * (R ** 3 + start + 10 ** 3 + 4 * 512 + 5 * 512)
         return {"kernel": "sor_uniform_kernel", "bound": "lds", "achieved": ach / 1e9, "peak": lds_peak / 1e9,
+                "resident_patches_per_cu": occ[1],
                 "unit": "GB/s", "frac": ach / lds_peak, "traffic": traffic.get(("sor_uniform_kernel", leg)),
                 "peak_basis": "measured LDS read bandwidth (pgamd_measure_peaks)" if peaks else "placeholder",
                 "frac_of_placeholder": ach / LDS_PEAK,
@@ -504,6 +577,7 @@ def main():
     asyn = None
     if a.asyn_streams and plain is not None:
         asyn = asyn_pipeline(a, pg, torch, ctx, rho, cpot, flu, args, plain["solve_ms"])
+    level = level_cycle(a, pg, ctx, args) if a.level_box > 0 else None
 
     def strip(r):
         return None if r is None else {k: v for k, v in r.items() if k != "iters"}
@@ -522,7 +596,7 @@ def main():
         "warm_vs_interp": (warm["value"] / interp["value"]) if (warm is not None and interp is not None) else None,
         "wall_ms_per_step": head_r["wall_ms_per_step"],
         "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu, "cpu_port": cpu_port,
-        "peaks_measured": peaks, "asyn_pipeline": asyn,
+        "peaks_measured": peaks, "asyn_pipeline": asyn, "level_cycle": level,
     }
     print(json.dumps(out))
     if net is not None:

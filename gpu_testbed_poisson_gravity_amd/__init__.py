@@ -20,6 +20,7 @@ INT_CQUAD, INT_QUAD = 4, 5
 INIT_INTERP, INIT_ZERO, INIT_GUESS, INIT_GUESS_CUBE = 0, 1, 2, 3
 TERM_REFERENCE, TERM_RELRES = 0, 1
 STATUS_CONVERGED, STATUS_MAX_ITER, STATUS_NONFINITE, STATUS_FALLBACK = 1, 2, 4, 8
+PACK_MISSING_FINE, PACK_MISSING_COARSE = 1, 2
 ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_NOMEM = -1, -2, -3, -4
 
 
@@ -103,8 +104,14 @@ SIGNATURES = {
                                                     C.c_int, C.c_int, C.c_int]),
     "pgamd_synchronize": (C.c_int, [_CTX]),
     "pgamd_get_patch_stats": (C.c_int, [_CTX, _IP, _IP, C.c_int]),
+    "pgamd_prepare_patch_groups": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_int]),
+    "pgamd_scatter_patch_groups": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                             C.c_void_p, C.c_void_p, C.c_int]),
     "pgamd_set_layout": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_int]),
     "pgamd_get_layout": (C.c_int, [_CTX, C.c_int, _IP, _IP]),
+    "pgamd_layout_occupancy": (C.c_int, [_CTX, C.c_int, _IP, _IP]),
     "pgamd_solve_device": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                      C.POINTER(SolverArgs), C.c_void_p, C.c_void_p, C.c_void_p]),
     "pgamd_solve_device_f64": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
@@ -298,6 +305,12 @@ class Context:
         _check(load().pgamd_get_layout(self.h, gs, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def layout_occupancy(self, gs: int):
+        """(workgroups per CU, patches per CU) of the SOR kernel at the current layout of ghost size gs."""
+        a, b = C.c_int(), C.c_int()
+        _check(load().pgamd_layout_occupancy(self.h, gs, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     # --- device-resident ----------------------------------------------------------------------------------
     def solve_device(self, rho, cpot, pot_out, flu, npatch, args: SolverArgs, guess=None, iters=None, status=None,
                      stream=None):
@@ -392,6 +405,64 @@ def pack(arrays, offsets, total):
         f = np.asarray(a, np.float32).ravel()
         out[o:o + f.size] = f
     return out
+
+
+def _c_array(a, dtype, shape=None, name="array"):
+    a = np.asarray(a)
+    if a.dtype != dtype or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a C-contiguous {np.dtype(dtype).name} array")
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"{name} has shape {a.shape}, expected {shape}")
+    return a
+
+
+def prepare_patch_groups(gs, group_first, group_father, fine_flu, fine_sib, coarse_pot, coarse_sib, h_rho=None,
+                         h_pot_in=None, h_flu=None, nthreads=0):
+    """pgamd_prepare_patch_groups: gather the solver's per-patch arrays for the given patch groups from a fine
+    level (fluid [nfine,5,8,8,8], neighbour table [nfine,27]) and its coarse level (potential [ncoarse,8,8,8],
+    neighbour table [ncoarse,27]).  Outputs may be given (e.g. the pinned buffers of Context.mem_allocate, as
+    flat float32 arrays); returns (rho [8g,R,R,R], pot_in [8g,C,C,C], flu [8g,5,8,8,8], status [8g])."""
+    g = geometry(gs)
+    gf = _c_array(group_first, np.int32, name="group_first")
+    ng = gf.shape[0]
+    gfa = _c_array(group_father, np.int32, (ng,), "group_father")
+    flu = _c_array(fine_flu, np.float32, name="fine_flu")
+    nfine = flu.shape[0]
+    _c_array(flu, np.float32, (nfine, 5, 8, 8, 8), "fine_flu")
+    fsib = _c_array(fine_sib, np.int32, (nfine, 27), "fine_sib")
+    cpot = _c_array(coarse_pot, np.float32, name="coarse_pot")
+    ncoarse = cpot.shape[0]
+    _c_array(cpot, np.float32, (ncoarse, 8, 8, 8), "coarse_pot")
+    csib = _c_array(coarse_sib, np.int32, (ncoarse, 27), "coarse_sib")
+    n = 8 * ng
+    out = []
+    for a, shp in ((h_rho, (n, g["R"], g["R"], g["R"])), (h_pot_in, (n, g["C"], g["C"], g["C"])),
+                   (h_flu, (n, 5, 8, 8, 8))):
+        if a is None:
+            a = np.empty(shp, np.float32)
+        elif np.asarray(a).size < int(np.prod(shp)) or np.asarray(a).dtype != np.float32:
+            raise ValueError("output array too small or not float32")
+        out.append(a)
+    st = np.zeros(n, np.int32)
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    _check(load().pgamd_prepare_patch_groups(gs, ng, _vp(gf), _vp(gfa), nfine, _vp(flu), _vp(fsib), ncoarse,
+                                             _vp(cpot), _vp(csib), _vp(out[0]), _vp(out[1]), _vp(out[2]), _vp(st),
+                                             nthreads))
+    return out[0], out[1], out[2], st
+
+
+def scatter_patch_groups(gg, group_first, h_pot_out, h_flu, fine_pot=None, fine_flu=None, nthreads=0):
+    """pgamd_scatter_patch_groups: write the patch part of the solver's potential output and the updated fluid
+    back into the fine level's arrays (in place); either may be None."""
+    gf = _c_array(group_first, np.int32, name="group_first")
+    nfine = (fine_pot if fine_pot is not None else fine_flu).shape[0]
+    if fine_pot is not None:
+        _c_array(fine_pot, np.float32, (nfine, 8, 8, 8), "fine_pot")
+    if fine_flu is not None:
+        _c_array(fine_flu, np.float32, (nfine, 5, 8, 8, 8), "fine_flu")
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    _check(load().pgamd_scatter_patch_groups(gg, gf.shape[0], _vp(gf), _vp(h_pot_out), _vp(h_flu), nfine,
+                                             _vp(fine_pot), _vp(fine_flu), nthreads))
 
 
 def solver_args_f64(dt=1e-2, dh=1.0, sor_min_iter=10, sor_max_iter=60, sor_omega=1.69, mg_max_iter=10, mg_npre=3,

@@ -418,6 +418,17 @@ int pgamd_get_layout(pgamd_context* c, int gs, int* ppw, int* pad) {
   return PGAMD_OK;
 }
 
+int pgamd_layout_occupancy(pgamd_context* c, int gs, int* wg, int* patches) {
+  if (!c) return fail(PGAMD_ERR_ARG, "null context");
+  if (gs < 1 || gs > kMaxGhost) return fail(PGAMD_ERR_ARG, "ghost size %d outside [1,5]", gs);
+  HIP_TRY(hipSetDevice(c->device));
+  const int n = sor_uniform_blocks_per_cu(gs, c->layout.ppw[gs], c->layout.pad[gs]);
+  if (n <= 0) return fail(PGAMD_ERR_DEVICE, "occupancy query failed");
+  if (wg) *wg = n;
+  if (patches) *patches = n * c->layout.ppw[gs];
+  return PGAMD_OK;
+}
+
 int pgamd_solve_device(pgamd_context* c, const float* d_rho, const float* d_pot_in, const float* d_guess,
                        float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* a, int* d_iters,
                        int* d_status, void* stream) {

@@ -148,6 +148,8 @@ hipError_t launch_sor_persistent(const MixedBatch& b, const ParamsByGs& p, int p
                                  hipStream_t stream);
 hipError_t launch_mg_persistent(const MixedBatch& b, const ParamsByGs& p, int grid, size_t lds, hipStream_t stream);
 int sor_persistent_blocks_per_cu(int ppw1, int ppw2, size_t lds);
+// Resident workgroups per CU of the uniform SOR kernel for a layout (0 if the layout does not exist).
+int sor_uniform_blocks_per_cu(int gs, int ppw, int pad);
 // Per-ghost-size launches of a mixed batch (the default): the SOR tasks [first, first + count) of ghost size
 // gs (ppw patches per wavefront, LDS row padding pad), and the MG tasks likewise.
 hipError_t launch_sor_tasks(const MixedBatch& b, const SolveParams& p, int gs, int ppw, int pad, int first, int count,

@@ -829,6 +829,39 @@ hipError_t launch_sor_uniform_dispatch(const UniformBatch& b, const SolveParams&
   }
 }
 
+template <int GS, int PPW, int PAD>
+static int sor_blocks_as() {
+  int n = 0;
+  const void* fn = reinterpret_cast<const void*>(&sor_uniform_kernel<layout_code(GS, PAD), PPW>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds_bytes_sor(GS, PPW, PAD)) != hipSuccess) return 0;
+  return n;
+}
+
+template <int GS, int PPW>
+static int sor_blocks_pad(int pad) {
+  switch (pad) {
+    case 0: return sor_blocks_as<GS, PPW, 0>();
+    case 1: return sor_blocks_as<GS, PPW, 1>();
+    case 2: return sor_blocks_as<GS, PPW, 2>();
+    case 3: return sor_blocks_as<GS, PPW, 3>();
+  }
+  return 0;
+}
+
+int sor_uniform_blocks_per_cu(int gs, int ppw, int pad) {
+  switch (gs * 8 + ppw) {
+    case 1 * 8 + 1: return sor_blocks_pad<1, 1>(pad);
+    case 1 * 8 + 2: return sor_blocks_pad<1, 2>(pad);
+    case 1 * 8 + 4: return sor_blocks_pad<1, 4>(pad);
+    case 2 * 8 + 1: return sor_blocks_pad<2, 1>(pad);
+    case 2 * 8 + 2: return sor_blocks_pad<2, 2>(pad);
+    case 3 * 8 + 1: return sor_blocks_pad<3, 1>(pad);
+    case 4 * 8 + 1: return sor_blocks_pad<4, 1>(pad);
+    case 5 * 8 + 1: return sor_blocks_pad<5, 1>(pad);
+  }
+  return 0;
+}
+
 hipError_t launch_uniform(const UniformBatch& b, const SolveParams& p, const SorLayout& lay, hipStream_t s) {
   if (b.npatch <= 0) return hipSuccess;
   if (b.scheme == kSchemeMG) return launch_mg_uniform(b, p, s);

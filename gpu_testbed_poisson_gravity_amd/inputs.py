@@ -288,3 +288,66 @@ def mixed_batch(npatch: int, seed: int, kinds=("lognormal",), gs_choices=(1, 2, 
             rho[i], cpot[i] = r[k], c[k]
         flu[sel] = f
     return gs, kind, rho, cpot, flu
+
+
+def _sib_table(coords: np.ndarray, extent: int) -> np.ndarray:
+    """Neighbour table [n, 27] of patches at integer coords [n, 3] (z, y, x) on a box of extent^3 patch slots:
+    entry (dz+1)*9 + (dy+1)*3 + (dx+1) is the index of the patch at coords + (dz, dy, dx), -1 where there is
+    none (outside the box; no periodicity)."""
+    slot = -np.ones((extent,) * 3, np.int64)
+    slot[coords[:, 0], coords[:, 1], coords[:, 2]] = np.arange(coords.shape[0])
+    padded = -np.ones((extent + 2,) * 3, np.int64)
+    padded[1:-1, 1:-1, 1:-1] = slot
+    sib = np.empty((coords.shape[0], 27), np.int32)
+    k = 0
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                sib[:, k] = padded[coords[:, 0] + 1 + dz, coords[:, 1] + 1 + dy, coords[:, 2] + 1 + dx]
+                k += 1
+    return sib
+
+
+def two_level_box(fine_flu_grid: np.ndarray, coarse_pot_grid: np.ndarray, refined_origin, seed: int = 0) -> dict:
+    """Two AMR levels in the patch-group format of pgamd_prepare_patch_groups, cut from global grids.
+
+    coarse_pot_grid [L, L, L] (L = 8 * coarse patches per side, spacing 2h) is the coarse level, covering the
+    whole domain.  The refined region is the box of m^3 coarse patches starting at coarse patch
+    refined_origin = (z, y, x); fine_flu_grid [5, 16m, 16m, 16m] (spacing h) is its fluid.  Each coarse patch
+    of the box has one patch group (8 fine patches).  Patch groups and coarse patches are stored in a seeded
+    random order, so nothing may rely on storage order.  Returns group_first, group_father, fine_flu
+    [8G, 5, 8, 8, 8], fine_sib, fine_coords [8G, 3] (fine patch coords inside the box, z y x), coarse_pot
+    [Nc, 8, 8, 8], coarse_sib, coarse_coords [Nc, 3]."""
+    rng = np.random.default_rng(seed)
+    L = coarse_pot_grid.shape[0]
+    if coarse_pot_grid.shape != (L, L, L) or L % PATCH:
+        raise ValueError("coarse_pot_grid must be [8k, 8k, 8k]")
+    ncp = L // PATCH
+    F = fine_flu_grid.shape[1]
+    if fine_flu_grid.shape != (5, F, F, F) or F % (2 * PATCH):
+        raise ValueError("fine_flu_grid must be [5, 16m, 16m, 16m]")
+    m = F // (2 * PATCH)
+    o = np.asarray(refined_origin, np.int64)
+    if np.any(o < 0) or np.any(o + m > ncp):
+        raise ValueError("refined box outside the coarse level")
+    # coarse patches in random order
+    cc = np.stack(np.meshgrid(*(np.arange(ncp),) * 3, indexing="ij"), -1).reshape(-1, 3)[rng.permutation(ncp ** 3)]
+    cpot = np.stack([coarse_pot_grid[z * 8:z * 8 + 8, y * 8:y * 8 + 8, x * 8:x * 8 + 8] for z, y, x in cc])
+    cindex = {tuple(c): i for i, c in enumerate(cc)}
+    # one patch group per coarse patch of the box, groups in random order, children in l = (lz*2+ly)*2+lx order
+    gb = np.stack(np.meshgrid(*(np.arange(m),) * 3, indexing="ij"), -1).reshape(-1, 3)[rng.permutation(m ** 3)]
+    fc = np.empty((8 * gb.shape[0], 3), np.int64)
+    for g, (z, y, x) in enumerate(gb):
+        for l in range(8):
+            fc[8 * g + l] = (2 * z + (l >> 2), 2 * y + ((l >> 1) & 1), 2 * x + (l & 1))
+    flu = np.stack([fine_flu_grid[:, z * 8:z * 8 + 8, y * 8:y * 8 + 8, x * 8:x * 8 + 8] for z, y, x in fc])
+    return {
+        "group_first": np.arange(0, 8 * gb.shape[0], 8, dtype=np.int32),
+        "group_father": np.asarray([cindex[tuple(b + o)] for b in gb], np.int32),
+        "fine_flu": np.ascontiguousarray(flu, np.float32),
+        "fine_sib": _sib_table(fc, 2 * m),
+        "fine_coords": fc,
+        "coarse_pot": np.ascontiguousarray(cpot, np.float32),
+        "coarse_sib": _sib_table(cc, ncp),
+        "coarse_coords": cc,
+    }

@@ -131,12 +131,46 @@ int pgamd_synchronize(pgamd_context* ctx);
  * (CPU_PoissonSolver_SOR.cpp:290). */
 int pgamd_get_patch_stats(pgamd_context* ctx, int* iters, int* status, int npatch);
 
+/* ---- patch-group preparation (host side; the "REPLACE in the actual implementation" step of
+ * GPU_API/CUAPI_Asyn_PoissonGravitySolver.cu:37-41 and CPU_Solver/CPU_PoissonSolver_SOR.cpp:5-7, 70-76: the
+ * testbed fills the arrays with synthetic data, Main.cpp:524, where GAMER gathers them from the AMR levels).
+ * No GPU involved; nthreads host threads (<= 1: the calling thread only).
+ *
+ * Level format.  Fine level: nfine patches of 8^3 cells, fine_flu [nfine][5][8][8][8] (rho, px, py, pz, E),
+ * fine_pot [nfine][8][8][8].  Patch group g: the 8 patches group_first[g] + l, l = (lz*2 + ly)*2 + lx, the
+ * children of coarse patch group_father[g] at offset (lx, ly, lz) * 8 fine cells.  Coarse level: coarse_pot
+ * [ncoarse][8][8][8] at twice the spacing.  Neighbour tables fine_sib [nfine][27], coarse_sib [ncoarse][27]:
+ * entry (dz+1)*9 + (dy+1)*3 + (dx+1) is the patch at offset (dx, dy, dz) in {-1,0,1}^3 (entry 13: the patch
+ * itself), -1 where there is none (outside the domain or the refined region).
+ *
+ * pgamd_prepare_patch_groups fills, for patch 8g + l of the batch, h_rho [R^3] with the fine density of the
+ * solve interior (the patch and pot_ghost_size - 1 cells of its neighbours), h_pot_in [C^3] with the coarse
+ * potential around the patch's own 4^3 coarse cells ((C - 4)/2 more on each side, from the father and its
+ * neighbours) and h_flu [5][8^3] with the patch's fluid: the layout of pgamd_asyn_poisson_gravity_solver.  Cells
+ * with no neighbour to take them from are set to 0 and flagged in status[8g + l] (may be NULL).
+ * PGAMD_ERR_ARG for sizes, null arrays, or indices outside the level. */
+#define PGAMD_PACK_MISSING_FINE 1     /* some density cells had no fine neighbour (set to 0) */
+#define PGAMD_PACK_MISSING_COARSE 2   /* some coarse potential cells had no coarse neighbour (set to 0) */
+int pgamd_prepare_patch_groups(int pot_ghost_size, int ngroup, const int* group_first, const int* group_father,
+                               int nfine, const float* fine_flu, const int* fine_sib, int ncoarse,
+                               const float* coarse_pot, const int* coarse_sib, float* h_rho, float* h_pot_in,
+                               float* h_flu, int* status, int nthreads);
+/* The reverse step (the reference's results stay in the arrays, Main.cpp:159-220): writes the patch part of
+ * h_pot_out [GRA_NXT^3] into fine_pot and h_flu into fine_flu for every patch of the ngroup groups.  Either
+ * destination may be NULL (not both). */
+int pgamd_scatter_patch_groups(int gra_ghost_size, int ngroup, const int* group_first, const float* h_pot_out,
+                               const float* h_flu, int nfine, float* fine_pot, float* fine_flu, int nthreads);
+
 /* Launch layout of the SOR kernel for one ghost size: patches per wavefront (1, 2 or 4 where they fit) and
  * the padding of the colour-split LDS rows (0..3 floats; trades LDS capacity for fewer bank conflicts).
  * This is the autotuner's search space (pgamd_autotune); no reference counterpart: the reference fixes its
  * launch shapes per GPU generation with macros (Header/CUPOT.h:38-152). */
 int pgamd_set_layout(pgamd_context* ctx, int ghost_size, int patches_per_wave, int row_pad);
 int pgamd_get_layout(pgamd_context* ctx, int ghost_size, int* patches_per_wave, int* row_pad);
+/* Residency of the SOR kernel at the context's layout for one ghost size: workgroups (one wavefront each) the
+ * device keeps resident per CU, and the patches that makes (times patches per wavefront), from the kernel's
+ * registers and LDS (hipOccupancyMaxActiveBlocksPerMultiprocessor).  No reference counterpart. */
+int pgamd_layout_occupancy(pgamd_context* ctx, int ghost_size, int* workgroups_per_cu, int* patches_per_cu);
 
 
 /* ---- device-resident entry points (inputs already in HBM; no reference counterpart: the reference

@@ -326,3 +326,21 @@ def test_full_cube_output_window(scheme):
         rho_d, cpot_d, out_d, flu_d = _dev_batch(gs, 8, gg=3)
         with pytest.raises(pg.PgError):  # the gravity stencil needs GRA_GHOST_SIZE <= 2
             ctx.solve_device(rho_d, cpot_d, out_d, flu_d, 8, pg.solver_args(graacc=True))
+
+
+def test_layout_occupancy():
+    """pgamd_layout_occupancy: resident SOR workgroups per CU at every ghost size and layout, consistent with the
+    layout's patches per wavefront; at ghost size 5 and one patch per wavefront, LDS (the colour-split cube) and
+    registers allow between 4 and 8 resident workgroups."""
+    with pg.Context(0) as ctx:
+        for gs in range(1, 6):
+            ppw, pad = ctx.get_layout(gs)
+            wg, patches = ctx.layout_occupancy(gs)
+            print(f"[occupancy] gs={gs} layout ppw={ppw} pad={pad}: {wg} workgroups, {patches} patches per CU",
+                  flush=True)
+            assert 1 <= wg <= 32 and patches == wg * ppw
+        ctx.set_layout(5, 1, 0)
+        wg, patches = ctx.layout_occupancy(5)
+        assert wg == patches and 4 <= wg <= 8
+        with pytest.raises(pg.PgError):
+            ctx.layout_occupancy(6)
