@@ -25,6 +25,11 @@ HBM_PEAK = 8.0e12          # B/s, MI355X published
 LDS_PEAK = 256 * 128 * 2.4e9  # B/s: 256 CUs x 128 B/clk x 2.4 GHz, used when the measurement is unavailable
 MFMA_BF16_PEAK = 2.5e15    # FLOP/s, dense BF16, MI355X published
 LDS_BYTES_PER_UPDATE = 36  # 7 phi reads + 1 rho read + 1 write, FP32
+# Arithmetic of one SOR cell update in the reference's operand order, no contraction: five adds of the
+# neighbours, 6*phi, two subtractions, omega/6 times the residual, and the add into phi.
+VALU_OPS_PER_UPDATE = 10
+# Vector issue rate without packed or fused operations: 256 CUs x 64 lanes per clock x 2.4 GHz (FP32 and FP64).
+VALU_PEAK = 256 * 64 * 2.4e9
 # Warm-start network FLOPs per patch that the model needs: five 3^3 convolutions 16 -> 16 channels and the
 # 16 -> 8 output layer on the 8^3 grid, K = 27 taps x 16 channels = 432:
 #   5 x 512 x 16 x 432 x 2 + 512 x 8 x 432 x 2 = 38,928,384.
@@ -73,7 +78,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--npatch", type=int, default=16384)
     ap.add_argument("--gs", type=int, default=5)
-    ap.add_argument("--check-every", type=int, default=1)
+    ap.add_argument("--check-every", type=int, default=2,
+                    help="relative-residual test every k iterations after Min_Iter (same rule on every leg)")
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="patches timed on the host cores by the reference CPU solver and by the oracle (0: skip; "
                          "default: the whole 16384-patch workload, about a second each on 16 host cores)")
@@ -98,6 +104,7 @@ def parse():
     ap.add_argument("--asyn-streams", default="1,2,4,8",
                     help="stream counts of the host-array pipeline leg (pgamd_asyn_poisson_gravity_solver with pinned "
                          "buffers, PCIe included; empty: skip)")
+    ap.add_argument("--fp64", type=int, default=1, help="1: also time the FP64 device path on the workload")
     ap.add_argument("--level-box", type=int, default=12,
                     help="refined box of M^3 coarse patches (8 M^3 fine patches) for the prepare/solve/scatter "
                          "cycle of two AMR levels; 0 skips it")
@@ -366,6 +373,9 @@ def level_cycle(a, pg, ctx, args):
             ts["scatter"].append(t3 - t2)
     it, st = ctx.patch_stats(n)
     ctx.mem_free()
+    dev_leg = level_cycle_device(a, pg, ctx, args, d, gs)
+    if a.level_box == 12 and gs == 5:
+        dev_leg["gather_roofline"]["traffic"] = measured_traffic(a).get(("pack_groups_kernel", "device"))
     ms = {k: float(np.median(v)) * 1e3 for k, v in ts.items()}
     tot = sum(ms.values())
     g = pg.geometry(gs)
@@ -374,10 +384,79 @@ def level_cycle(a, pg, ctx, args):
            "prepare_ms": ms["prepare"], "solve_ms": ms["solve"], "scatter_ms": ms["scatter"], "total_ms": tot,
            "patch_solves_per_s": n / (tot * 1e-3), "prepare_GBps": moved / (ms["prepare"] * 1e-3) / 1e9,
            "mean_iters": float(it.mean()), "converged": int(np.sum((st & pg.STATUS_CONVERGED) != 0)),
-           "note": "host-side AMR cycle (gather, PCIe-inclusive solve, scatter): never the headline value"}
+           "note": "host-side AMR cycle (gather, PCIe-inclusive solve, scatter): never the headline value",
+           "device": dev_leg}
     log(f"[bench] level cycle ({n} fine patches, {nt} host threads): prepare {ms['prepare']:.2f} ms, solve "
         f"{ms['solve']:.2f} ms, scatter {ms['scatter']:.2f} ms -> {out['patch_solves_per_s']:.4g} patch-solves/s")
     return out
+
+
+def level_cycle_device(a, pg, ctx, args, d, gs):
+    """The same cycle with both levels resident in HBM: pack_groups_kernel gathers the batch
+    (pgamd_prepare_patch_groups_device), the solve runs on it (interpolated start, the context's rule),
+    scatter_groups_kernel writes potential and fluid back.  Each phase timed with events on one stream.
+    Gather roofline (HBM): per patch it writes (R^3 + C^3 + 5*8^3) floats and must read at least the patch's
+    fluid (5*8^3) and its share of the coarse level (8^3/8), each once."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    g = pg.geometry(gs)
+    ng = d["group_first"].size
+    n = 8 * ng
+    t = {k: torch.as_tensor(np.ascontiguousarray(v)).to(dev) for k, v in d.items()
+         if k not in ("fine_coords", "coarse_coords")}
+    nfine, ncoarse = t["fine_flu"].shape[0], t["coarse_pot"].shape[0]
+    rho = torch.empty((n, g["R"], g["R"], g["R"]), device=dev)
+    pin = torch.empty((n, g["C"], g["C"], g["C"]), device=dev)
+    flu = torch.empty((n, 5, 8, 8, 8), device=dev)
+    pst = torch.empty(n, dtype=torch.int32, device=dev)
+    out = torch.empty((n, 10, 10, 10), device=dev)
+    it = torch.empty(n, dtype=torch.int32, device=dev)
+    st = torch.empty(n, dtype=torch.int32, device=dev)
+    fine_pot = torch.zeros((nfine, 8, 8, 8), device=dev)
+    fine_flu = t["fine_flu"].clone()
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+
+    def step(e=None):
+        if e is not None:
+            e[0].record(stream)
+        pg.prepare_patch_groups_device(gs, ng, t["group_first"], t["group_father"], nfine, t["fine_flu"],
+                                       t["fine_sib"], ncoarse, t["coarse_pot"], t["coarse_sib"], rho, pin, flu, pst,
+                                       stream=sp)
+        if e is not None:
+            e[1].record(stream)
+        ctx.solve_device(rho, pin, out, flu, n, args, iters=it, status=st, stream=sp)
+        if e is not None:
+            e[2].record(stream)
+        pg.scatter_patch_groups_device(1, ng, t["group_first"], out, flu, nfine, fine_pot, fine_flu, stream=sp)
+        if e is not None:
+            e[3].record(stream)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize(dev)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(a.steps)]
+    for k in range(a.steps):
+        step(evs[k])
+    torch.cuda.synchronize(dev)
+    ph = [float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs])) for i in range(3)]
+    written = n * 4.0 * (g["R"] ** 3 + g["C"] ** 3 + 5 * 512)
+    minread = n * 4.0 * (5 * 512 + 512 / 8)
+    gather_alg = written + minread
+    tot = sum(ph)
+    res = {"prepare_ms": ph[0], "solve_ms": ph[1], "scatter_ms": ph[2], "total_ms": tot,
+           "patch_solves_per_s": n / (tot * 1e-3), "mean_iters": float(it.float().mean().item()),
+           "pack_flags": int((pst != 0).sum().item()),
+           "gather_roofline": {"kernel": "pack_groups_kernel", "bound": "hbm", "algorithmic_bytes": gather_alg,
+                               "achieved": gather_alg / (ph[0] * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                               "unit": "GB/s", "frac": gather_alg / (ph[0] * 1e-3) / HBM_PEAK,
+                               "bytes_basis": "written (R^3 + C^3 + 5*8^3 floats) + minimum read (5*8^3 + 8^3/8) "
+                                              "per patch"},
+           "note": "levels resident in HBM: gather kernel, solve, scatter kernel; nothing crosses PCIe"}
+    log(f"[bench] level cycle on the device: gather {ph[0]:.3f} ms, solve {ph[1]:.3f} ms, scatter {ph[2]:.3f} ms "
+        f"-> {res['patch_solves_per_s']:.4g} patch-solves/s; gather {res['gather_roofline']['achieved']:.0f} GB/s")
+    return res
 
 
 def main():
@@ -483,7 +562,11 @@ def main():
         start = {"plain": g["C"] ** 3, "interp": g["C"] ** 3}.get(
             leg, N ** 3 if a.guess == "cube" else R ** 3 + g["C"] ** 3)
         hbm = a.npatch * 4.0 * (R ** 3 + start + 10 ** 3 + 4 * 512 + 5 * 512)
+        ops = float(r["iters"].astype(np.float64).sum()) * R ** 3 * VALU_OPS_PER_UPDATE
         return {"kernel": "sor_uniform_kernel", "bound": "lds", "achieved": ach / 1e9, "peak": lds_peak / 1e9,
+                "valu": {"ops_per_update": VALU_OPS_PER_UPDATE, "achieved_Tops": ops / (r["solve_ms"] * 1e-3) / 1e12,
+                         "peak_Tops": VALU_PEAK / 1e12, "frac": ops / (r["solve_ms"] * 1e-3) / VALU_PEAK,
+                         "basis": "FP32 arithmetic of the updates alone; the kernel's real limit (no packed FP32)"},
                 "resident_patches_per_cu": occ[1],
                 "unit": "GB/s", "frac": ach / lds_peak, "traffic": traffic.get(("sor_uniform_kernel", leg)),
                 "peak_basis": "measured LDS read bandwidth (pgamd_measure_peaks)" if peaks else "placeholder",
@@ -500,6 +583,33 @@ def main():
         # method without the network
         interp = run(pg.INIT_INTERP, None)
         log(f"[bench] interp: {interp['value']:.4g} patch-solves/s, {interp['mean_iters']:.1f} iterations")
+    fp64 = None
+    if a.fp64 and a.mode in ("both", "plain"):
+        # the FP64 device path (the reference's FLOAT8 build) on the same workload, plain leg's settings
+        ctx.set_options(pot_ghost_size=a.gs, gra_ghost_size=1, pot_scheme=pg.SOR, init_mode=pg.INIT_ZERO,
+                        term_mode=pg.TERM_RELRES, check_every=a.check_every, rel_tol=1e-5, fuse_gravity=1,
+                        guess_fallback_tol=0.0)
+        a64 = pg.solver_args_f64(dt=1e-3, sor_min_iter=10, sor_max_iter=1000, sor_omega=1.8, poi_coeff=poi)
+        r64, c64, f64 = (x.double() for x in (d_rho, d_cpot, d_flu0))
+        fl64 = f64.clone()
+        o64 = torch.empty((a.npatch, 10, 10, 10), dtype=torch.float64, device=dev)
+        steps64 = max(2, a.steps // 4)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        for k in range(1 + steps64):
+            fl64.copy_(f64)
+            if k == 1:
+                ev[0].record(stream)
+            ctx.solve_device_f64(r64, c64, o64, fl64, a.npatch, a64, iters=d_it, status=d_st, stream=sp)
+        ev[1].record(stream)
+        torch.cuda.synchronize(dev)
+        ms64 = ev[0].elapsed_time(ev[1]) / steps64      # includes the D2D fluid restore (0.05 ms at 2.7 TB/s)
+        it64 = d_it.cpu().numpy()
+        fp64 = {"value": a.npatch / (ms64 * 1e-3), "ms_per_step": ms64, "mean_iters": float(it64.mean()),
+                "iters_sum": float(it64.astype(np.float64).sum()),
+                "converged": int(((d_st.cpu().numpy() & pg.STATUS_CONVERGED) != 0).sum()), "steps": steps64,
+                "note": "sor64_kernel, zero guess to relative residual 1e-5 in FP64; the plain leg in FP64"}
+        log(f"[bench] fp64 plain: {fp64['value']:.4g} patch-solves/s, {fp64['mean_iters']:.1f} iterations")
+        del r64, c64, f64, fl64, o64
     net = None
     if a.mode in ("both", "warm"):
         if os.path.exists(a.weights):
@@ -524,6 +634,14 @@ def main():
     roofs = {"sor": roof_sor}
     if warm_serial is not None and plain is not None:
         roofs["sor_plain"] = lds_roof(plain, "plain")
+    if interp is not None:
+        roofs["sor_interp"] = lds_roof(interp, "interp")
+    if fp64 is not None:
+        ops64 = fp64["iters_sum"] * R ** 3 * VALU_OPS_PER_UPDATE
+        roofs["sor64"] = {"kernel": "sor64_kernel", "bound": "valu", "achieved": ops64 / (fp64["ms_per_step"] * 1e-3) / 1e12,
+                          "peak": VALU_PEAK / 1e12, "unit": "Tops/s (FP64)",
+                          "frac": ops64 / (fp64["ms_per_step"] * 1e-3) / VALU_PEAK,
+                          "traffic": traffic.get(("sor64_kernel", "fp64")), "launch_ms": fp64["ms_per_step"]}
     roof = roof_sor
     if warm_serial is not None:
         flop = CNN_USEFUL_FLOP * a.npatch
@@ -596,7 +714,7 @@ def main():
         "warm_vs_interp": (warm["value"] / interp["value"]) if (warm is not None and interp is not None) else None,
         "wall_ms_per_step": head_r["wall_ms_per_step"],
         "roofline": roof, "roofline_kernels": roofs, "cpu_baseline": cpu, "cpu_port": cpu_port,
-        "peaks_measured": peaks, "asyn_pipeline": asyn, "level_cycle": level,
+        "peaks_measured": peaks, "asyn_pipeline": asyn, "level_cycle": level, "fp64": fp64,
     }
     print(json.dumps(out))
     if net is not None:

@@ -20,7 +20,7 @@ INT_CQUAD, INT_QUAD = 4, 5
 INIT_INTERP, INIT_ZERO, INIT_GUESS, INIT_GUESS_CUBE = 0, 1, 2, 3
 TERM_REFERENCE, TERM_RELRES = 0, 1
 STATUS_CONVERGED, STATUS_MAX_ITER, STATUS_NONFINITE, STATUS_FALLBACK = 1, 2, 4, 8
-PACK_MISSING_FINE, PACK_MISSING_COARSE = 1, 2
+PACK_MISSING_FINE, PACK_MISSING_COARSE, PACK_BAD_INDEX = 1, 2, 4
 ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_NOMEM = -1, -2, -3, -4
 
 
@@ -109,6 +109,11 @@ SIGNATURES = {
                                              C.c_void_p, C.c_void_p, C.c_int]),
     "pgamd_scatter_patch_groups": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                              C.c_void_p, C.c_void_p, C.c_int]),
+    "pgamd_prepare_patch_groups_device": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                                    C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pgamd_scatter_patch_groups_device": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                                    C.c_void_p, C.c_void_p, C.c_void_p]),
     "pgamd_set_layout": (C.c_int, [_CTX, C.c_int, C.c_int, C.c_int]),
     "pgamd_get_layout": (C.c_int, [_CTX, C.c_int, _IP, _IP]),
     "pgamd_layout_occupancy": (C.c_int, [_CTX, C.c_int, _IP, _IP]),
@@ -463,6 +468,23 @@ def scatter_patch_groups(gg, group_first, h_pot_out, h_flu, fine_pot=None, fine_
     nthreads = nthreads or min(16, os.cpu_count() or 1)
     _check(load().pgamd_scatter_patch_groups(gg, gf.shape[0], _vp(gf), _vp(h_pot_out), _vp(h_flu), nfine,
                                              _vp(fine_pot), _vp(fine_flu), nthreads))
+
+
+def prepare_patch_groups_device(gs, ngroup, group_first, group_father, nfine, fine_flu, fine_sib, ncoarse,
+                                coarse_pot, coarse_sib, rho, pot_in, flu=None, status=None, stream=None):
+    """pgamd_prepare_patch_groups_device: the gather on device-resident levels (device pointers or tensors, the
+    layouts of prepare_patch_groups; int32 index tables).  Asynchronous on `stream`."""
+    _check(load().pgamd_prepare_patch_groups_device(gs, ngroup, _vp(group_first), _vp(group_father), nfine,
+                                                    _vp(fine_flu), _vp(fine_sib), ncoarse, _vp(coarse_pot),
+                                                    _vp(coarse_sib), _vp(rho), _vp(pot_in), _vp(flu), _vp(status),
+                                                    stream))
+
+
+def scatter_patch_groups_device(gg, ngroup, group_first, pot_out, flu, nfine, fine_pot=None, fine_flu=None,
+                                stream=None):
+    """pgamd_scatter_patch_groups_device: the write-back into device-resident levels.  Asynchronous."""
+    _check(load().pgamd_scatter_patch_groups_device(gg, ngroup, _vp(group_first), _vp(pot_out), _vp(flu), nfine,
+                                                    _vp(fine_pot), _vp(fine_flu), stream))
 
 
 def solver_args_f64(dt=1e-2, dh=1.0, sor_min_iter=10, sor_max_iter=60, sor_omega=1.69, mg_max_iter=10, mg_npre=3,

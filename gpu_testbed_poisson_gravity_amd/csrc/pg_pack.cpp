@@ -43,7 +43,7 @@ inline void split(int u, int& d, int& w) {
 // One x row of a packed array: n cells starting at x = u0 relative to the patch origin (-8 <= u0, u0 + n <= 16),
 // taken in up to three runs from the patches nb[0], nb[1], nb[2] (offsets dx = -1, 0, 1); row_of(Q) is the start
 // of the same (z, y) row in patch Q.  Runs without a patch are zero-filled and return `missing`; an index at or
-// beyond `count` marks the group bad.
+// beyond `count` zero-fills its run, returns PGAMD_PACK_BAD_INDEX and marks the group bad.
 template <typename RowOf>
 int gather_row(float* dst, int u0, int n, const int* nb, int count, int& bad, int missing, RowOf&& row_of) {
   int flag = 0;
@@ -54,9 +54,10 @@ int gather_row(float* dst, int u0, int n, const int* nb, int count, int& bad, in
     const int Q = nb[dx + 1];
     if (Q < 0) {
       std::fill(out, out + (hi - lo), 0.0f);
-      flag = missing;
+      flag |= missing;
     } else if (Q >= count) {
       std::fill(out, out + (hi - lo), 0.0f);
+      flag |= PGAMD_PACK_BAD_INDEX;
       bad = 1;
     } else {
       std::memcpy(out, row_of(Q) + (lo - kP * dx), sizeof(float) * (hi - lo));

@@ -52,7 +52,8 @@ extern "C" {
  * CPU_PoissonSolver_SOR.cpp:273-283.  The device sums |R| as a wavefront tree, the reference CPU
  * sequentially: where two successive totals are equal to within rounding, the stall can be detected an
  * iteration or a few earlier or later than on the reference CPU (results then differ at FP32 round-off;
- * at equal iteration counts they are bitwise equal).  RELRES: stop when sum|R| <= rel_tol * sum|phi| over the interior,
+ * at equal iteration counts they are bitwise equal).  Measured on the 1320 stall-rule patches of the
+ * reference's own outputs (ghost sizes 1..5, CQUAD and QUAD): every patch stops at the reference's iteration.  RELRES: stop when sum|R| <= rel_tol * sum|phi| over the interior,
  * tested after max(1,Min_Iter) iterations and then every check_every iterations.  MG always uses the
  * reference's relative error dh^2 sum|Def| / sum|Sol| <= MG_Tolerated_Error. */
 #define PGAMD_TERM_REFERENCE 0
@@ -148,9 +149,11 @@ int pgamd_get_patch_stats(pgamd_context* ctx, int* iters, int* status, int npatc
  * potential around the patch's own 4^3 coarse cells ((C - 4)/2 more on each side, from the father and its
  * neighbours) and h_flu [5][8^3] with the patch's fluid: the layout of pgamd_asyn_poisson_gravity_solver.  Cells
  * with no neighbour to take them from are set to 0 and flagged in status[8g + l] (may be NULL).
- * PGAMD_ERR_ARG for sizes, null arrays, or indices outside the level. */
+ * PGAMD_ERR_ARG for sizes, null arrays, or indices outside the level (a neighbour index beyond the level is
+ * also flagged PGAMD_PACK_BAD_INDEX in status). */
 #define PGAMD_PACK_MISSING_FINE 1     /* some density cells had no fine neighbour (set to 0) */
 #define PGAMD_PACK_MISSING_COARSE 2   /* some coarse potential cells had no coarse neighbour (set to 0) */
+#define PGAMD_PACK_BAD_INDEX 4        /* a group or neighbour index outside the level (cells set to 0) */
 int pgamd_prepare_patch_groups(int pot_ghost_size, int ngroup, const int* group_first, const int* group_father,
                                int nfine, const float* fine_flu, const int* fine_sib, int ncoarse,
                                const float* coarse_pot, const int* coarse_sib, float* h_rho, float* h_pot_in,
@@ -160,6 +163,21 @@ int pgamd_prepare_patch_groups(int pot_ghost_size, int ngroup, const int* group_
  * destination may be NULL (not both). */
 int pgamd_scatter_patch_groups(int gra_ghost_size, int ngroup, const int* group_first, const float* h_pot_out,
                                const float* h_flu, int nfine, float* fine_pot, float* fine_flu, int nthreads);
+/* The same two steps for levels resident in device memory (no reference counterpart: GAMER keeps its levels
+ * on the host).  All pointers are device pointers; one kernel launch each on `stream` (a hipStream_t, NULL =
+ * the default stream) of the current device; asynchronous.  The fluid arrays must be 16-byte aligned.  The
+ * outputs are bitwise those of the host versions.  Index errors cannot be returned from the device: a patch
+ * whose group or neighbour indices lie outside the level gets PGAMD_PACK_BAD_INDEX in d_status (its invalid
+ * cells, or the whole patch for a bad group, set to 0), and the scatter skips groups whose group_first is out of
+ * range. */
+int pgamd_prepare_patch_groups_device(int pot_ghost_size, int ngroup, const int* d_group_first,
+                                      const int* d_group_father, int nfine, const float* d_fine_flu,
+                                      const int* d_fine_sib, int ncoarse, const float* d_coarse_pot,
+                                      const int* d_coarse_sib, float* d_rho, float* d_pot_in, float* d_flu,
+                                      int* d_status, void* stream);
+int pgamd_scatter_patch_groups_device(int gra_ghost_size, int ngroup, const int* d_group_first,
+                                      const float* d_pot_out, const float* d_flu, int nfine, float* d_fine_pot,
+                                      float* d_fine_flu, void* stream);
 
 /* Launch layout of the SOR kernel for one ghost size: patches per wavefront (1, 2 or 4 where they fit) and
  * the padding of the colour-split LDS rows (0..3 floats; trades LDS capacity for fewer bank conflicts).
