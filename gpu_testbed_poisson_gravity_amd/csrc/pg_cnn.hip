@@ -566,12 +566,10 @@ __global__ void __launch_bounds__(kThreads) guess_kernel(const float* __restrict
 // * Wave w owns output rows w and w + 4 of every plane and sweeps the 8 input planes once; every A operand
 //   read from LDS feeds up to three MFMAs, and the next plane's operands are requested before the current
 //   plane's MFMAs.
-// * The output layer's epilogue adds s * D to the interpolated interior (kept in LDS) and stores the guess.
-// LDS per workgroup: three buffers (76.8 KB), two workgroups per CU.  Measured on MI355X against computing
-// layers 2-5 in place in one buffer (51.2 KB, three workgroups per CU, a barrier per plane, the VGPR budget cut
-// to 168): 0.832 vs 0.877 ms per 16384 patches, so the extra occupancy does not pay for the barriers and the
-// register cap.
-
+// * Two buffers only: the coarse staging, the packed input and the activations share them with the FP32
+//   interpolated cube.  The interpolated values the guess adds to are written to the guess array while the
+//   input is packed; the output layer's epilogue reads them back (same lane, same address: L2), adds s * D and
+//   stores the guess.  LDS per workgroup 51.2 KB: three workgroups per CU.
 constexpr int kHalfV = S * S * 8;      // bf16 per half-plane (10 x 10 voxels x 8 channels)
 constexpr int kPlaneV = 2 * kHalfV;    // bf16 per plane
 constexpr int kActVec = So * kPlaneV / 8;   // uint4 per buffer (8 planes)
@@ -581,12 +579,11 @@ constexpr int kInteriorMax = 16 * 16 * 16;  // R^3 at ghost size 5
 constexpr int kFT = 256;                    // threads of a fused workgroup (4 waves)
 constexpr int kRing = 4 * S - 4;            // ring voxels of a half-plane (36)
 
-constexpr int kFusedBlocksPerCU = 2;
+constexpr int kFusedBlocksPerCU = 3;
 
 struct FusedLds {
-  uint4 X[kActVec];   // coarse staging -> packed input -> interpolated interior (FP32) for the guess
+  uint4 X[kActVec];   // coarse staging -> packed input -> activations
   uint4 C[kActVec];   // interpolated cube (FP32, unpadded N^3) -> activations
-  uint4 B[kActVec];   // activations
   float red[kFT / 64];
 };
 static_assert(kCoarseMax * 4 <= kActVec * 16 && kInteriorMax * 4 <= kActVec * 16 && kCubeMax * 4 <= kActVec * 16,
@@ -631,8 +628,8 @@ __device__ __forceinline__ void zero_ring(uint4* buf) {
 // What the output layer's epilogue needs: the scale, the interpolated interior (LDS) and where the guess goes.
 struct GuessOut {
   float s;
-  const float* in;   // interpolated interior, [R^3] FP32
-  float* guess;      // [R^3] (cube = false) or the whole solve cube [N^3] (cube = true, interior written here)
+  float* guess;      // [R^3] (cube = false) or the whole solve cube [N^3] (cube = true): holds the interpolated
+                     // values on entry to the output layer; the epilogue adds s * D in place
 };
 
 // Epilogue of output plane o for this wave's row pair: output channels co0..co0+3 (co0 = 4*(lane/16)) of voxel
@@ -656,12 +653,8 @@ __device__ __forceinline__ void conv_store(const u16* in, u16* out, const floatx
       for (int k = 0; k < 4; k++) {
         const int ly = ly0 + (k >> 1), lx = lx0 + (k & 1);
         if (lz >= 0 && lz < R && ly >= 0 && ly < R && lx >= 0 && lx < R) {
-          const int t = (lz * R + ly) * R + lx;
-          const float v = go.in[t] + go.s * h[k];
-          if (CUBE)
-            go.guess[((lz + 1) * N + ly + 1) * N + lx + 1] = v;
-          else
-            go.guess[t] = v;
+          float* gp = go.guess + (CUBE ? ((lz + 1) * N + ly + 1) * N + lx + 1 : (lz * R + ly) * R + lx);
+          *gp = *gp + go.s * h[k];
         }
       }
     }
@@ -761,11 +754,8 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   float* cube = reinterpret_cast<float*>(L.C);
   u16* X0 = reinterpret_cast<u16*>(L.X);
   u16* A = reinterpret_cast<u16*>(L.C);
-  // first layer's weights, the coarse cube and this thread's density column are requested together.  Thread t
-  // owns the frame-interior column (fx, fy) = (t % 16, t / 16); its solve-interior column is (fx, fy) - OFF.
-  bf16x8 w[KZ];
-  float4 b;
-  load_layer(Wb, P, 0, w, b);
+  // the coarse cube and this thread's density column are requested together.  Thread t owns the frame-interior
+  // column (fx, fy) = (t % 16, t / 16); its solve-interior column is (fx, fy) - OFF.
   for (int t = threadIdx.x; t < C3 / 4; t += kFT)
     reinterpret_cast<float4*>(cs)[t] = reinterpret_cast<const float4*>(cpot)[t];
   const int fx = threadIdx.x & 15, fy = threadIdx.x >> 4;
@@ -802,13 +792,14 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   uint32_t* X32 = reinterpret_cast<uint32_t*>(X0);
   const int colw = (((fy >> 1) + 1) * S + (fx >> 1) + 1) * 4 + (fy & 1) * 2 + (fx & 1);
   auto word = [&](int fz) -> uint32_t& { return X32[(fz >> 1) * (kPlaneV / 2) + (fz & 1) * (kHalfV / 2) + colw]; };
-  float icol[R];
+  zero_ring(L.X);
 #pragma unroll
   for (int fz = 0; fz < OFF; fz++) word(fz) = 0u;
 #pragma unroll
   for (int fz = OFF + R; fz < 16; fz++) word(fz) = 0u;
   if (colv) {
     const float* cc = cube + (y + 1) * N + (x + 1);   // cube column, stride N^2
+    float* gcol = guess + (CUBE ? N * N + (y + 1) * N + (x + 1) : y * R + x);   // interior column, z = 0
     float cm = cc[0], c = cc[N * N];
 #pragma unroll
     for (int z = 0; z < R; z++) {
@@ -816,33 +807,26 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
       const float cp = cz[N * N];
       const float r = cp + cm + cz[N] + cz[-N] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
       word(z + OFF) = pack2_bf16(c * inv, (r * inv) * kResScale);
-      icol[z] = c;
+      gcol[z * (CUBE ? N * N : R * R)] = c;
       cm = c;
       c = cp;
     }
   } else {
 #pragma unroll
-    for (int z = 0; z < R; z++) {
-      word(z + OFF) = 0u;
-      icol[z] = 0.0f;
-    }
+    for (int z = 0; z < R; z++) word(z + OFF) = 0u;
   }
-  zero_ring(L.X);
-  zero_ring(L.B);
+  bf16x8 w[KZ];
+  float4 b;
+  load_layer(Wb, P, 0, w, b);
   __syncthreads();   // the packed input is complete; the cube is dead
   zero_ring(L.C);
   int off[5];
   pair_offsets(off);
   __syncthreads();
-  GuessOut go{s, reinterpret_cast<const float*>(L.X), guess};
+  GuessOut go{s, guess};
   conv_layer<GS, CUBE, 0>(X0, A, w, b, off, go);   // layer 1: packed input -> A
   __syncthreads();   // every wave is done reading the packed input
-  if (colv) {
-    float* in = reinterpret_cast<float*>(L.X);
-#pragma unroll
-    for (int z = 0; z < R; z++) in[(z * R + y) * R + x] = icol[z];
-  }
-  u16* Bb = reinterpret_cast<u16*>(L.B);
+  u16* Bb = X0;
   load_layer(Wb, P, 1, w, b);
   conv_layer<GS, CUBE, 1>(A, Bb, w, b, off, go);   // layers 2-5: A -> B -> A -> B -> A
   __syncthreads();
