@@ -620,8 +620,8 @@ def main():
                 f"{warm_serial['mean_iters']:.1f} iterations, network {warm_serial['network_ms']:.3f} ms + "
                 f"solve {warm_serial['solve_ms']:.3f} ms")
             warm = warm_serial
-            if a.chunks > 0 and a.guess == "interior":
-                warm_pipe = run(pg.INIT_GUESS, net, pipelined=True)
+            if a.chunks > 0:
+                warm_pipe = run(pg.INIT_GUESS_CUBE if a.guess == "cube" else pg.INIT_GUESS, net, pipelined=True)
                 log(f"[bench] warm (pipelined, {a.chunks} slices): {warm_pipe['value']:.4g} patch-solves/s, "
                     f"{warm_pipe['mean_iters']:.1f} iterations")
                 if warm_pipe["value"] > warm["value"]:

@@ -624,7 +624,7 @@ class CNN:
 
     def solve_device(self, rho, cpot, guess, pot_out, flu, npatch, args: SolverArgs, iters=None, status=None,
                      nchunk=4, stream=None):
-        """Warm start + solve, pipelined over nchunk slices (context init_mode must be INIT_GUESS)."""
+        """Warm start + solve, pipelined over nchunk slices (context init_mode INIT_GUESS or INIT_GUESS_CUBE)."""
         _check(load().pgamd_cnn_solve_device(self.ctx.h, self.h, _vp(rho), _vp(cpot), _vp(guess), _vp(pot_out),
                                              _vp(flu), npatch, C.byref(args), _vp(iters), _vp(status), nchunk,
                                              stream))

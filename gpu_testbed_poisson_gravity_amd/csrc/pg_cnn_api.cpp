@@ -372,8 +372,8 @@ int pgamd_cnn_solve_device(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho,
     return set_error(PGAMD_ERR_ARG, "null argument");
   if (npatch < 0 || nchunk < 1 || nchunk > kMaxChunks) return set_error(PGAMD_ERR_ARG, "bad patch or chunk count");
   const pgamd_options& o = ctx_options(ctx);
-  if (o.init_mode != PGAMD_INIT_GUESS || !a->poisson)
-    return set_error(PGAMD_ERR_ARG, "the warm-start solve needs init_mode GUESS and poisson = 1");
+  if ((o.init_mode != PGAMD_INIT_GUESS && o.init_mode != PGAMD_INIT_GUESS_CUBE) || !a->poisson)
+    return set_error(PGAMD_ERR_ARG, "the warm-start solve needs init_mode GUESS or GUESS_CUBE and poisson = 1");
   if (a->int_scheme != PGAMD_INT_CQUAD && a->int_scheme != PGAMD_INT_QUAD)
     return set_error(PGAMD_ERR_ARG, "incorrect parameter IntScheme");
   if (npatch == 0) return PGAMD_OK;
@@ -381,6 +381,8 @@ int pgamd_cnn_solve_device(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho,
   const Geo g = geo_of(o.pot_ghost_size);
   const size_t R3 = (size_t)g.R * g.R * g.R, C3 = (size_t)g.C * g.C * g.C;
   const size_t G = gra_nxt(o.gra_ghost_size), G3 = G * G * G, F = (size_t)kNComp * kPatchCells;
+  const int cube = o.init_mode == PGAMD_INIT_GUESS_CUBE;
+  const size_t Q3 = cube ? (size_t)g.N * g.N * g.N : R3;   // guess per patch
   hipStream_t s0 = (hipStream_t)stream, s1 = ctx_aux_stream(ctx);
   if (!s1) return set_error(PGAMD_ERR_DEVICE, "auxiliary stream creation failed");
   for (hipEvent_t& e : n->ev)
@@ -393,10 +395,10 @@ int pgamd_cnn_solve_device(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho,
   for (int i = 0, off = 0; off < npatch; i++, off += chunk) {
     const int m = std::min(chunk, npatch - off);
     CK(cnn_fused_guess(d_rho + off * R3, d_pot_in + off * C3, n->wb, n->p, m, o.pot_ghost_size, cst, a->int_scheme,
-                       d_guess + off * R3, 0, s1));
+                       d_guess + off * Q3, cube, s1));
     CK(hipEventRecord(n->ev[i], s1));
     CK(hipStreamWaitEvent(s0, n->ev[i], 0));
-    const int rc = pgamd_solve_device(ctx, d_rho + off * R3, d_pot_in + off * C3, d_guess + off * R3,
+    const int rc = pgamd_solve_device(ctx, d_rho + off * R3, d_pot_in + off * C3, d_guess + off * Q3,
                                       d_pot_out + off * G3, d_flu ? d_flu + off * F : nullptr, m, a,
                                       d_iters ? d_iters + off : nullptr, d_status ? d_status + off : nullptr, s0);
     if (rc != PGAMD_OK) return rc;

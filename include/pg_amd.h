@@ -321,8 +321,9 @@ int pgamd_cnn_warm_start_cube(pgamd_context* ctx, pgamd_cnn* net, const float* d
                               int npatch, const pgamd_solver_args* args, float* d_guess_cube, void* stream);
 /* Warm start and solve of a uniform batch in one call, pipelined over nchunk (1..16) slices: the network
  * of slice i+1 runs on the context's second stream while slice i is solved on `stream`.  The context must
- * have init_mode PGAMD_INIT_GUESS (guess_fallback_tol applies); d_guess ([n][R^3]) is workspace and
- * receives the guesses.  Same results as pgamd_cnn_warm_start + pgamd_solve_device. */
+ * have init_mode PGAMD_INIT_GUESS or PGAMD_INIT_GUESS_CUBE (guess_fallback_tol applies); d_guess ([n][R^3],
+ * or [n][N^3] for the whole cube) is workspace and receives the guesses.  Same results as
+ * pgamd_cnn_warm_start (_cube) + pgamd_solve_device. */
 int pgamd_cnn_solve_device(pgamd_context* ctx, pgamd_cnn* net, const float* d_rho, const float* d_pot_in,
                            float* d_guess, float* d_pot_out, float* d_flu, int npatch, const pgamd_solver_args* args,
                            int* d_iters, int* d_status, int nchunk, void* stream);
