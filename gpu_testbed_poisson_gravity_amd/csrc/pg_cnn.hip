@@ -628,8 +628,7 @@ __device__ __forceinline__ void zero_ring(uint4* buf) {
 // What the output layer's epilogue needs: the scale, the interpolated interior (LDS) and where the guess goes.
 struct GuessOut {
   float s;
-  float* guess;      // [R^3] (cube = false) or the whole solve cube [N^3] (cube = true): holds the interpolated
-                     // values on entry to the output layer; the epilogue adds s * D in place
+  float* sd;         // s * D on the solve interior, [R^3] FP32 in the LDS buffer the output layer does not read
 };
 
 // Epilogue of output plane o for this wave's row pair: output channels co0..co0+3 (co0 = 4*(lane/16)) of voxel
@@ -645,17 +644,13 @@ __device__ __forceinline__ void conv_store(const u16* in, u16* out, const floatx
   const float h0 = acc[0] + b.x, h1 = acc[1] + b.y, h2 = acc[2] + b.z, h3 = acc[3] + b.w;
   if (MODE == 2) {
     if (co0 < kCnnOut) {
-      constexpr Geo g = geo_of(GS);
-      constexpr int N = g.N, R = g.R, OFF = 5 - GS;
+      constexpr int R = geo_of(GS).R, OFF = 5 - GS;
       const int lz = 2 * o + (co0 >> 2) - OFF, ly0 = 2 * row - OFF, lx0 = 2 * xv - OFF;
       const float h[4] = {h0, h1, h2, h3};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const int ly = ly0 + (k >> 1), lx = lx0 + (k & 1);
-        if (lz >= 0 && lz < R && ly >= 0 && ly < R && lx >= 0 && lx < R) {
-          float* gp = go.guess + (CUBE ? ((lz + 1) * N + ly + 1) * N + lx + 1 : (lz * R + ly) * R + lx);
-          *gp = *gp + go.s * h[k];
-        }
+        if (lz >= 0 && lz < R && ly >= 0 && ly < R && lx >= 0 && lx < R) go.sd[(lz * R + ly) * R + lx] = go.s * h[k];
       }
     }
   } else {
@@ -823,7 +818,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   int off[5];
   pair_offsets(off);
   __syncthreads();
-  GuessOut go{s, guess};
+  GuessOut go{s, reinterpret_cast<float*>(L.X)};   // X is free during the output layer (layer 5 read it)
   conv_layer<GS, CUBE, 0>(X0, A, w, b, off, go);   // layer 1: packed input -> A
   __syncthreads();   // every wave is done reading the packed input
   u16* Bb = X0;
@@ -840,7 +835,19 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   conv_layer<GS, CUBE, 1>(Bb, A, w, b, off, go);
   __syncthreads();
   load_layer(Wb, P, 5, w, b);
-  conv_layer<GS, CUBE, 2>(A, nullptr, w, b, off, go);   // layer 6: the guess
+  conv_layer<GS, CUBE, 2>(A, nullptr, w, b, off, go);   // layer 6: s * D into X
+  __syncthreads();
+  // the guess, interp + s * D, by interior columns: the interpolated values parked in the guess array are read
+  // and the sums written back as whole rows
+  if (colv) {
+    const float* sd = reinterpret_cast<const float*>(L.X) + y * R + x;
+    float* gcol = guess + (CUBE ? N * N + (y + 1) * N + (x + 1) : y * R + x);
+#pragma unroll
+    for (int z = 0; z < R; z++) {
+      float* gp = gcol + z * (CUBE ? N * N : R * R);
+      *gp = *gp + sd[z * R * R];
+    }
+  }
 }
 
 template <bool CUBE>
