@@ -792,9 +792,12 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   for (int fz = 0; fz < OFF; fz++) word(fz) = 0u;
 #pragma unroll
   for (int fz = OFF + R; fz < 16; fz++) word(fz) = 0u;
+  // The column's interpolated values are kept for the guess: in registers for the whole-cube guess (164 VGPRs),
+  // parked in the guess array otherwise (the interior and mixed-batch instances would spill).
+  float icol[R];
+  float* gcol = guess + (CUBE ? N * N + (y + 1) * N + (x + 1) : y * R + x);   // interior column, z = 0
   if (colv) {
     const float* cc = cube + (y + 1) * N + (x + 1);   // cube column, stride N^2
-    float* gcol = guess + (CUBE ? N * N + (y + 1) * N + (x + 1) : y * R + x);   // interior column, z = 0
     float cm = cc[0], c = cc[N * N];
 #pragma unroll
     for (int z = 0; z < R; z++) {
@@ -802,7 +805,10 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
       const float cp = cz[N * N];
       const float r = cp + cm + cz[N] + cz[-N] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
       word(z + OFF) = pack2_bf16(c * inv, (r * inv) * kResScale);
-      gcol[z * (CUBE ? N * N : R * R)] = c;
+      if (CUBE)
+        icol[z] = c;
+      else
+        gcol[z * R * R] = c;
       cm = c;
       c = cp;
     }
@@ -837,15 +843,15 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   load_layer(Wb, P, 5, w, b);
   conv_layer<GS, CUBE, 2>(A, nullptr, w, b, off, go);   // layer 6: s * D into X
   __syncthreads();
-  // the guess, interp + s * D, by interior columns: the interpolated values parked in the guess array are read
-  // and the sums written back as whole rows
+  // the guess, interp + s * D, by interior columns (whole rows per wave)
   if (colv) {
     const float* sd = reinterpret_cast<const float*>(L.X) + y * R + x;
-    float* gcol = guess + (CUBE ? N * N + (y + 1) * N + (x + 1) : y * R + x);
 #pragma unroll
     for (int z = 0; z < R; z++) {
-      float* gp = gcol + z * (CUBE ? N * N : R * R);
-      *gp = *gp + sd[z * R * R];
+      if (CUBE)
+        gcol[z * N * N] = icol[z] + sd[z * R * R];
+      else
+        gcol[z * R * R] = gcol[z * R * R] + sd[z * R * R];
     }
   }
 }
