@@ -445,8 +445,9 @@ def prepare_patch_groups(gs, group_first, group_father, fine_flu, fine_sib, coar
                    (h_flu, (n, 5, 8, 8, 8))):
         if a is None:
             a = np.empty(shp, np.float32)
-        elif np.asarray(a).size < int(np.prod(shp)) or np.asarray(a).dtype != np.float32:
-            raise ValueError("output array too small or not float32")
+        elif not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous
+                  and a.size >= int(np.prod(shp))):
+            raise ValueError(f"output arrays must be C-contiguous float32 numpy arrays of at least {shp}")
         out.append(a)
     st = np.zeros(n, np.int32)
     nthreads = nthreads or min(16, os.cpu_count() or 1)
@@ -460,7 +461,15 @@ def scatter_patch_groups(gg, group_first, h_pot_out, h_flu, fine_pot=None, fine_
     """pgamd_scatter_patch_groups: write the patch part of the solver's potential output and the updated fluid
     back into the fine level's arrays (in place); either may be None."""
     gf = _c_array(group_first, np.int32, name="group_first")
+    if fine_pot is None and fine_flu is None:
+        raise ValueError("nothing to scatter: fine_pot and fine_flu are both None")
     nfine = (fine_pot if fine_pot is not None else fine_flu).shape[0]
+    n, G = 8 * gf.shape[0], 8 + 2 * gg
+    for a, need, name in ((h_pot_out if fine_pot is not None else None, n * G ** 3, "h_pot_out"),
+                          (h_flu if fine_flu is not None else None, n * 5 * 512, "h_flu")):
+        if a is not None and not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous
+                                  and a.size >= need):
+            raise ValueError(f"{name} must be a C-contiguous float32 array of at least {need} values")
     if fine_pot is not None:
         _c_array(fine_pot, np.float32, (nfine, 8, 8, 8), "fine_pot")
     if fine_flu is not None:

@@ -178,3 +178,24 @@ def test_pack_error_paths():
     assert lib.pgamd_prepare_patch_groups(gs, 0, None, None, 0, None, None, 0, None, None, None, None, None, None,
                                           1) == 0
     assert lib.pgamd_scatter_patch_groups(1, 0, None, None, None, 0, None, None, 1) == 0
+
+
+def test_python_binding_checks_buffers():
+    """The binding refuses buffers the C side would overrun: too small, non-contiguous, or of the wrong type."""
+    gs = 2
+    _, _, d = _level(2, 1, (0, 0, 0), seed=6)
+    g = pg.geometry(gs)
+    n = 8 * d["group_first"].size
+    with pytest.raises(ValueError):
+        _prepare(gs, d, h_rho=np.zeros(n * g["R"] ** 3 - 1, np.float32))
+    with pytest.raises(ValueError):
+        _prepare(gs, d, h_rho=np.zeros((n * g["R"] ** 3, 2), np.float32)[:, 0])    # strided view
+    with pytest.raises(ValueError):
+        _prepare(gs, d, h_flu=np.zeros(n * 5 * 512, np.float64))
+    fine_pot = np.zeros((d["fine_flu"].shape[0], 8, 8, 8), np.float32)
+    with pytest.raises(ValueError):
+        pg.scatter_patch_groups(1, d["group_first"], np.zeros(n * 1000 - 1, np.float32), None, fine_pot=fine_pot)
+    with pytest.raises(ValueError):
+        pg.scatter_patch_groups(1, d["group_first"], None, None)
+    pg.scatter_patch_groups(1, d["group_first"], np.zeros(n * 1000, np.float32), None, fine_pot=fine_pot)
+    assert np.all(fine_pot == 0)
