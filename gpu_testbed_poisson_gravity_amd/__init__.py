@@ -21,6 +21,7 @@ INIT_INTERP, INIT_ZERO, INIT_GUESS, INIT_GUESS_CUBE = 0, 1, 2, 3
 TERM_REFERENCE, TERM_RELRES = 0, 1
 STATUS_CONVERGED, STATUS_MAX_ITER, STATUS_NONFINITE, STATUS_FALLBACK = 1, 2, 4, 8
 PACK_MISSING_FINE, PACK_MISSING_COARSE, PACK_BAD_INDEX = 1, 2, 4
+LAYOUT_COMPACT = 4   # SOR layout code of the compact single-buffer cube (ghost size 5, one patch per wavefront)
 ERR_ARG, ERR_DEVICE, ERR_STATE, ERR_NOMEM = -1, -2, -3, -4
 
 

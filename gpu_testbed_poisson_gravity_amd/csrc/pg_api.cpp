@@ -404,7 +404,9 @@ int pgamd_set_layout(pgamd_context* c, int gs, int ppw, int pad) {
   if (!c) return fail(PGAMD_ERR_ARG, "null context");
   if (gs < 1 || gs > kMaxGhost) return fail(PGAMD_ERR_ARG, "ghost size %d outside [1,5]", gs);
   if (!supported_ppw(gs, ppw)) return fail(PGAMD_ERR_ARG, "%d patches per wavefront do not fit ghost size %d", ppw, gs);
-  if (pad < 0 || pad > 3) return fail(PGAMD_ERR_ARG, "row padding %d outside [0,3]", pad);
+  if (!supported_layout(gs, ppw, pad))
+    return fail(PGAMD_ERR_ARG, "row padding %d is neither 0..3 nor the compact layout (ghost size 5, one patch "
+                "per wavefront)", pad);
   c->layout.ppw[gs] = ppw;
   c->layout.pad[gs] = pad;
   return PGAMD_OK;

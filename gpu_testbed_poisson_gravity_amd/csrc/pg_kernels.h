@@ -8,6 +8,10 @@
 
 namespace pgamd {
 
+// SOR layout code for the compact single-buffer LDS layout (ghost size 5, one patch per wavefront), in place
+// of a split-row padding of 0..3 floats.  Same value as PGAMD_LAYOUT_COMPACT of the C ABI.
+constexpr int kPadCompact = 4;
+
 // Uniform per-launch solver settings.  Floating-point constants are formed once on the host, in FP32, the
 // way the reference CPU solver forms them (CPU_PoissonSolver_SOR.cpp:51-52), so that device and CPU agree
 // bit for bit.
@@ -134,7 +138,7 @@ struct ParamsByGs {
 // the default is the densest packing that fits one wavefront.
 struct SorLayout {
   int ppw[6];              // patches per wavefront, indexed by gs
-  int pad[6];              // split-row padding in floats (bank-conflict avoidance), indexed by gs
+  int pad[6];              // split-row padding in floats (bank-conflict avoidance), or kPadCompact, by gs
   int waves_per_cu;        // persistent kernel occupancy target
 };
 
@@ -161,6 +165,7 @@ hipError_t launch_reset_counters(unsigned int* counters, int n, hipStream_t stre
 int sor_lanes_per_patch(int gs);
 int default_ppw(int gs);
 int default_pad(int gs);
+bool supported_layout(int gs, int ppw, int pad);
 size_t lds_bytes_sor(int gs, int ppw, int pad);
 size_t lds_bytes_mg(int gs);
 int supported_ppw(int gs, int ppw);

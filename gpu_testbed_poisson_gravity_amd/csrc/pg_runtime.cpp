@@ -389,7 +389,7 @@ int pgamd_graph_destroy(pgamd_graph* g) {
 // ---- autotuner ------------------------------------------------------------------------------------------
 //
 // Search space per ghost size: patches per wavefront in {1, 2, 4} (where they fit the 64 lanes) x row
-// padding in {0..3} floats.  Each candidate runs one untimed solve and `repeats` timed ones on the caller's
+// padding in {0..3} floats, plus the compact layout at ghost size 5 (one patch per wavefront).  Each candidate runs one untimed solve and `repeats` timed ones on the caller's
 // stream-less default queue; the fastest mean wins.  The table is cached as a small JSON document:
 //   {"device": "<name>", "layouts": {"1": [ppw, pad], ..., "5": [ppw, pad]}}
 
@@ -426,7 +426,8 @@ int pgamd_autotune(pgamd_context* c, int gs, const float* rho, const float* cpot
   UniformBatch b{rho, cpot, nullptr, pot_out, nullptr, nullptr, nullptr, npatch, gs, PGAMD_SOR};
   for (int ppw : {1, 2, 4}) {
     if (!supported_ppw(gs, ppw)) continue;
-    for (int pad = 0; pad <= 3; pad++) {
+    for (int pad = 0; pad <= kPadCompact; pad++) {
+      if (!supported_layout(gs, ppw, pad)) continue;
       hipError_t e = launch_sor_uniform_dispatch(b, p, ppw, pad, st);
       if (e != hipSuccess) continue;  // layout does not launch on this device (e.g. LDS): skip it
       hipEventRecord(e0, st);
@@ -493,7 +494,7 @@ int pgamd_autotune_load(pgamd_context* c, const char* path) {
     const std::string key = "\"" + std::to_string(g) + "\": [";
     const size_t k = s.find(key);
     if (k == std::string::npos || sscanf(s.c_str() + k + key.size(), "%d, %d", &ppw[g], &pad[g]) != 2 ||
-        !supported_ppw(g, ppw[g]) || pad[g] < 0 || pad[g] > 3)
+        !supported_layout(g, ppw[g], pad[g]))
       return set_error(PGAMD_ERR_ARG, "malformed autotune cache");
   }
   SorLayout& lay = ctx_layout(c);

@@ -50,7 +50,14 @@ struct SorGeo {
   static constexpr int HP = N / 2 + PAD;        // split row stride in floats
   static constexpr int MX = R / 2;              // x-pairs per row
   static constexpr int LP = MX * (R / 2);       // lanes per patch
-  static constexpr int FLOATS = 2 * N * N * HP; // LDS floats per patch
+  static constexpr bool COMPACT = PAD == kPadCompact;   // single-buffer layout (ghost size 5 only)
+  static constexpr int FLOATS = COMPACT ? 16 * 256 : 2 * N * N * HP;   // LDS floats per patch
+  // Lane order within a patch.  Row-major (lane = yb*MX + m) puts consecutive lanes on consecutive words of a
+  // split row; x-major (lane = m*MX + yb) puts them 2*HP words apart.  An LDS access serves 32 lanes per
+  // cycle over 32 four-byte banks, so x-major at HP = 10 (ghost size 5, one float of padding) sends the 32
+  // lanes of a half-wave (yb = 0..7, four values of m) to words 20*yb + m: 32 distinct banks.  Measured on
+  // MI355X (16384 patches): x-major is faster at ghost sizes 4 and 5, equal or slower below.
+  static constexpr bool XMAJOR = gs >= 4;
 };
 
 constexpr int layout_code(int gs, int pad) { return gs + 8 * pad; }
@@ -211,15 +218,83 @@ struct HalfSweep {
   static constexpr int xx(int q) { return (COL + (q & 1) + (q >> 1) + 1) & 1; }
 };
 
-// The two LDS operands of owned cell q: the neighbour across the y-pair and the one across the x-pair,
-// both in the other colour's array at row offset m+xx.
-template <int GS, int COL>
-__device__ __forceinline__ void load_nb(const float* __restrict__ lds, int lb, int q, float (&v)[2]) {
-  using H = HalfSweep<GS, COL>;
-  const int yy = H::yy(q), zi = H::zi(q), xx = H::xx(q);
-  v[0] = lds[lb + own_off<GS>(1 - COL, zi + 1, yy == 0 ? -1 : 2, xx)];
-  v[1] = lds[lb + own_off<GS>(1 - COL, zi + 1, yy, xx)];
-}
+// LDS addressing of a lane's cells.  load_nb gives the two LDS operands of owned cell q (v[0]: the neighbour
+// across the y-pair, v[1]: the one across the x-pair); own gives where the lane publishes cell q.
+//
+// Split layout: both colours resident, the lane's operands at row offset m+xx of the other colour's array.
+template <int GS>
+struct SplitAddr {
+  static constexpr int kLook = kLookahead;
+  int lb;   // 2*yb*HP + m
+  template <int COL>
+  __device__ __forceinline__ void load_nb(const float* __restrict__ lds, int q, float (&v)[2]) const {
+    using H = HalfSweep<GS, COL>;
+    const int yy = H::yy(q), zi = H::zi(q), xx = H::xx(q);
+    v[0] = lds[lb + own_off<GS>(1 - COL, zi + 1, yy == 0 ? -1 : 2, xx)];
+    v[1] = lds[lb + own_off<GS>(1 - COL, zi + 1, yy, xx)];
+  }
+  template <int COL>
+  __device__ __forceinline__ int own(int q) const {
+    using H = HalfSweep<GS, COL>;
+    return lb + own_off<GS>(COL, H::zi(q) + 1, H::yy(q), H::xx(q));
+  }
+};
+
+// Compact layout (ghost size 5, one patch per wavefront).  A red-black half-sweep reads only the other
+// colour and then overwrites nothing but its own colour, so one buffer can hold, for each interior cell pair
+// x in {2s-1, 2s} of a row, whichever of the two cells was written last: a half-sweep reads the other colour
+// there and then publishes its own colour into the same slots.  Plane z (1..16) starts at (z-1)*kPS floats;
+// interior row y (1..16) at (y-1)*kRS with slot 0 the Dirichlet cell x = 0, slots 1..8 the pairs and slot 9
+// the Dirichlet cell x = 17; the Dirichlet rows y = 0 and 17 keep their 16 interior cells (x = 2m+1 at A+m,
+// x = 2m+2 at B+m).  The z = 0 and 17 planes stay in the lanes' registers.  16 KB per patch instead of the
+// split layout's 25.9 KB: eight resident patches per CU instead of six.
+// Banks: the 32 lanes of an LDS cycle (yb = 0..7, four values of m) read rows 20*yb + m + const, 28 of the 32
+// four-byte banks for a y-neighbour read; A and B sit on the 4 banks left free (23..26 mod 32 for y = 0, 1..4
+// for y = 17).  kPS = 256 is a multiple of 64 floats, so two planes pair into one ds_read2st64 from the
+// lane's base: the sweeps need no address arithmetic.
+struct Compact {
+  static constexpr int kRS = 10, kPS = 256, kPlanes = 16, kFloats = kPlanes * kPS;
+  static constexpr int kY0A = 183, kY0B = 215, kY17A = 161, kY17B = 193;
+  // slot of cube cell (x, y, z), x, y, z in 1..16 plus the Dirichlet cells x or y in {0, 17}, z in 1..16
+  __device__ static constexpr int slot(int x, int y, int z) {
+    return (z - 1) * kPS + (y == 0 ? (((x - 1) & 1) ? kY0B : kY0A) + ((x - 1) >> 1)
+                            : y == 17 ? (((x - 1) & 1) ? kY17B : kY17A) + ((x - 1) >> 1)
+                                      : (y - 1) * kRS + ((x + 1) >> 1));
+  }
+};
+
+// Five lane registers address every operand: the lane's row base B = 20*yb + m (x-neighbours at B + 10*yy +
+// 2*xx, its own slots at B + 10*yy + 1, compile-time offsets from one register) and the y-neighbour bases of
+// the two rows, which move to the Dirichlet rows for the lanes at yb = 0 (below) and yb = 7 (above).
+template <int GS>
+struct CompactAddr {
+  static constexpr int kLook = 4;
+  int b;          // 20*yb + m
+  int ylo[2];     // y-neighbour below (yy = 0) of the lane's cell xx, plane 1
+  int yhi[2];     // y-neighbour above (yy = 1)
+  __device__ __forceinline__ void init(int m, int yb) {
+    b = 2 * yb * Compact::kRS + m;
+#pragma unroll
+    for (int xx = 0; xx < 2; xx++) {
+      const int x = 2 * m + 1 + xx;
+      ylo[xx] = Compact::slot(x, 2 * yb, 1);
+      yhi[xx] = Compact::slot(x, 2 * yb + 3, 1);
+    }
+    asm volatile("" : "+v"(b), "+v"(ylo[0]), "+v"(ylo[1]), "+v"(yhi[0]), "+v"(yhi[1]));
+  }
+  template <int COL>
+  __device__ __forceinline__ void load_nb(const float* __restrict__ lds, int q, float (&v)[2]) const {
+    using H = HalfSweep<GS, COL>;
+    const int yy = H::yy(q), zi = H::zi(q), xx = H::xx(q);
+    v[0] = lds[(yy == 0 ? ylo[xx] : yhi[xx]) + Compact::kPS * zi];
+    v[1] = lds[b + Compact::kRS * yy + 2 * xx + Compact::kPS * zi];
+  }
+  template <int COL>
+  __device__ __forceinline__ int own(int q) const {
+    using H = HalfSweep<GS, COL>;
+    return b + Compact::kRS * H::yy(q) + 1 + Compact::kPS * H::zi(q);
+  }
+};
 
 // Pre-update residual of owned cell q of colour COL, reference operand order
 // (CPU_PoissonSolver_SOR.cpp:247-250): ((((((kp+km)+jp)+jm)+ip)+im) - 6 phi) - Const*rho.
@@ -238,13 +313,12 @@ __device__ __forceinline__ float residual_at(const LaneCells<GS>& L, const float
 }
 
 // Publish the lane's cells of colour COL to LDS (one exec-masked block per half-sweep).
-template <int GS, int COL>
-__device__ __forceinline__ void store_colour(const LaneCells<GS>& L, float* __restrict__ lds, int lb, bool active) {
+template <int GS, int COL, class A>
+__device__ __forceinline__ void store_colour(const LaneCells<GS>& L, float* __restrict__ lds, const A& a, bool active) {
   using H = HalfSweep<GS, COL>;
   if (active) {
 #pragma unroll
-    for (int q = 0; q < H::NQ; q++)
-      lds[lb + own_off<GS>(COL, H::zi(q) + 1, H::yy(q), H::xx(q))] = L.phi[H::yy(q)][H::xx(q)][H::zi(q)];
+    for (int q = 0; q < H::NQ; q++) lds[a.template own<COL>(q)] = L.phi[H::yy(q)][H::xx(q)][H::zi(q)];
   }
 }
 
@@ -252,18 +326,19 @@ __device__ __forceinline__ void store_colour(const LaneCells<GS>& L, float* __re
 // omega/6 for running patches and 0 for converged ones: phi + 0*R == phi for the finite residuals of a
 // converged patch, so no branch is needed).  MODE 1: keep R in res[] and leave phi unchanged.
 // Returns sum|R| when ACC.
-template <int GS, int COL, int MODE, bool ACC>
-__device__ __forceinline__ float half_sweep(LaneCells<GS>& L, const float* __restrict__ lds, int lb, float w,
+template <int GS, int COL, int MODE, bool ACC, class A>
+__device__ __forceinline__ float half_sweep(LaneCells<GS>& L, const float* __restrict__ lds, const A& a, float w,
                                             float (&res)[2 * SorGeo<GS>::R]) {
   using H = HalfSweep<GS, COL>;
   constexpr int NQ = H::NQ;
   float nb[NQ][2];
+  constexpr int LA = A::kLook;
 #pragma unroll
-  for (int q = 0; q < kLookahead && q < NQ; q++) load_nb<GS, COL>(lds, lb, q, nb[q]);
+  for (int q = 0; q < LA && q < NQ; q++) a.template load_nb<COL>(lds, q, nb[q]);
   float acc = 0.0f;
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    if (q + kLookahead < NQ) load_nb<GS, COL>(lds, lb, q + kLookahead, nb[q + kLookahead]);
+    if (q + LA < NQ) a.template load_nb<COL>(lds, q + LA, nb[q + LA]);
     const float r = residual_at<GS, COL>(L, nb[q], q);
     if (ACC) acc += fabsf(r);
     if (MODE == 0) {
@@ -309,11 +384,10 @@ struct LanePatch {
 // Gravity update of one patch from the potential held in an LDS cube (CPU_GravitySolver.cpp:34-79).
 // The SEG lanes serving the patch take cells lane, lane+SEG, ...; the fluid of up to 8 cells per lane is
 // requested before any of them is updated, so a lane waits for memory once per chunk, not once per cell.
-template <int GS, int SEG>
-__device__ void gravity_from_lds(const float* __restrict__ lds, float* __restrict__ flu, int lane, float gra_const,
-                                 int p5) {
+template <int GS, int SEG, class Idx>
+__device__ void gravity_from_lds(const float* __restrict__ lds, Idx idx, float* __restrict__ flu, int lane,
+                                 float gra_const, int p5) {
   using S = SorGeo<GS>;
-  constexpr int N = S::N, HP = S::HP;
   constexpr int NT = kPatchCells / SEG;          // cells per lane
   constexpr int CHUNK = NT < 8 ? NT : 8;
   static_assert(kPatchCells % SEG == 0 && NT % CHUNK == 0, "lane segment must divide the patch");
@@ -330,7 +404,7 @@ __device__ void gravity_from_lds(const float* __restrict__ lds, float* __restric
       const int t = lane + (c0 + u) * SEG;
       const int i = t % kPatchSize, j = (t / kPatchSize) % kPatchSize, k = t / (kPatchSize * kPatchSize);
       const int x = i + S::gs, y = j + S::gs, z = k + S::gs;
-      auto P = [&](int dz, int dy, int dx) { return lds[cube_index(N, HP, x + dx, y + dy, z + dz)]; };
+      auto P = [&](int dz, int dy, int dx) { return lds[idx(x + dx, y + dy, z + dz)]; };
       float ax, ay, az;
       if (p5) {
         ax = gra_const * (-P(0, 0, 2) + P(0, 0, -2) + 8.0f * P(0, 0, 1) - 8.0f * P(0, 0, -1));
@@ -356,43 +430,14 @@ __device__ void gravity_from_lds(const float* __restrict__ lds, float* __restric
   }
 }
 
-// Solve up to PPW patches of ghost size GS with one wavefront (blockDim.x == 64).
-template <int GS, int PPW>
-__device__ __forceinline__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, const SolveParams& P) {
-  using S = SorGeo<GS>;
-  constexpr int N = S::N, R = S::R, HP = S::HP, MX = S::MX, LP = S::LP;
-  constexpr int SEG = 64 / PPW;
-  static_assert(LP <= SEG, "patch does not fit its lane segment");
-  int lane = threadIdx.x;
-  // opaque per patch: inside the persistent kernel's task loop the compiler would otherwise hoist every
-  // lane-derived address out of the loop and keep them all live (measured: 2 KB of spills per lane)
-  asm volatile("" : "+v"(lane));
-  const int slot = lane / SEG, pl = lane % SEG;
-  float* lds = lds_wave + slot * S::FLOATS;
-  const bool active = lp.valid && pl < LP;
-  const int plc = pl < LP ? pl : LP - 1;  // idle lanes mirror a real lane's addresses and never write
-  const int m = plc % MX, yb = plc / MX;
-  const int lb = 2 * yb * HP + m;
-
-  // A. coarse -> fine interpolation of the whole cube (initial guess and Dirichlet layer); a zero start or a
-  // caller's guess needs only the Dirichlet layer (a rejected guess restarts from zero, not from the
-  // interpolation), and a whole-cube guess brings its own Dirichlet layer
-  if (lp.valid) {
-    if (P.init_mode == 3)
-      shell_to_lds<GS, SEG>(lp.guess, lds, pl);
-    else if (P.init_mode != 0)
-      interp_to_lds<GS, true>(lp.cpot, lds, pl, SEG, P.int_scheme);
-    else
-      interp_to_lds<GS, false>(lp.cpot, lds, pl, SEG, P.int_scheme);
-  }
-  __syncthreads();
-
-  // B. owned cells into registers
-  // The lane's two x-neighbours (xx = 0, 1) are adjacent in memory: one 8-byte load reads both, so every
-  // load instruction covers whole cache lines (no line is requested twice).  All density and guess loads of
-  // the lane are issued before any is used: one memory round trip for the phase.
-  LaneCells<GS> L;
-  float2 r2[2][R], g2[2][R];
+// Owned density (and a caller's guess) of a lane's 2x2 columns.  The lane's two x-neighbours (xx = 0, 1) are
+// adjacent in memory: one 8-byte load reads both, so one load instruction of a patch reads R/2 whole rows of
+// the cube (the same rows in either lane order).  All loads are issued before any is used: one memory round
+// trip for the phase.
+template <int GS>
+__device__ __forceinline__ void load_owned(const LanePatch& lp, const SolveParams& P, int m, int yb, bool active,
+                                           float2 (&r2)[2][SorGeo<GS>::R], float2 (&g2)[2][SorGeo<GS>::R]) {
+  constexpr int N = SorGeo<GS>::N, R = SorGeo<GS>::R;
 #pragma unroll
   for (int yy = 0; yy < 2; yy++)
 #pragma unroll
@@ -419,6 +464,372 @@ __device__ __forceinline__ void sor_task(float* __restrict__ lds_wave, const Lan
         }
     }
   }
+}
+
+// Reject a caller guess whose relative residual exceeds guess_tol: restart from the zero guess.  Both colours'
+// residuals are taken from the same field.  The compact layout holds one colour at a time: the first colour is
+// published between the two residual passes, and the second colour (the zero guess after a rejection) is
+// published again afterwards, as the first sweep expects.
+template <int GS, int SEG, bool COMPACT, class A>
+__device__ __forceinline__ bool guess_check(LaneCells<GS>& L, float* __restrict__ lds, const A& a, const SolveParams& P,
+                                            bool valid, bool active, float (&res)[2 * SorGeo<GS>::R]) {
+  constexpr int R = SorGeo<GS>::R;
+  float num = half_sweep<GS, 1, 1, true>(L, lds, a, 0.0f, res);
+  if (COMPACT) {
+    store_colour<GS, 1>(L, lds, a, active);
+    __syncthreads();
+  }
+  num += half_sweep<GS, 0, 1, true>(L, lds, a, 0.0f, res);
+  float den = 0.0f;
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+      for (int zi = 0; zi < R; zi++) den += fabsf(L.phi[yy][xx][zi]);
+  if (!active) num = den = 0.0f;
+  num = seg_sum<SEG>(num);
+  den = seg_sum<SEG>(den);
+  const bool fell_back = valid && !(num <= P.guess_tol * den);
+  if (fell_back) {
+#pragma unroll
+    for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+      for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+        for (int zi = 0; zi < R; zi++) L.phi[yy][xx][zi] = 0.0f;
+  }
+  if (COMPACT) {
+    store_colour<GS, 0>(L, lds, a, active);
+  } else {
+    store_colour<GS, 1>(L, lds, a, active && fell_back);
+    store_colour<GS, 0>(L, lds, a, active && fell_back);
+  }
+  __syncthreads();
+  return fell_back;
+}
+
+// Red-black iterations of one patch per lane segment.  Sets it_done and st (converged or max_iter).
+template <int GS, int SEG, class A>
+__device__ __forceinline__ void sor_iterate(LaneCells<GS>& L, float* __restrict__ lds, const A& a, const SolveParams& P,
+                                            bool active, float (&res)[2 * SorGeo<GS>::R], int& it_done, int& st) {
+  constexpr int R = SorGeo<GS>::R;
+  bool running = active;
+  it_done = P.max_iter;
+  st = kStatusMaxIter;
+  const bool relres = P.term_mode == 1, stall = P.term_mode == 0;
+  const int first_check = P.min_iter > 1 ? P.min_iter : 1;
+  float sum_b = 0.0f, old = FLT_MAX;
+  for (int t = 0; t < P.max_iter; ++t) {
+    const bool check = relres && t >= first_check && (t - first_check) % P.check_every == 0;
+    const bool pre_check = relres && t + 1 >= first_check && (t + 1 - first_check) % P.check_every == 0;
+    float tot = 0.0f;
+    if (check) {
+      float num = half_sweep<GS, 1, 1, true>(L, lds, a, 0.0f, res), den = 0.0f;
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+          for (int zi = 0; zi < R; zi++) den += fabsf(L.phi[yy][xx][zi]);
+      num = num + P.damp * sum_b;
+      if (!active) num = den = 0.0f;
+      num = seg_sum<SEG>(num);
+      den = seg_sum<SEG>(den);
+      if (running && num <= P.rel_tol * den) {
+        running = false;
+        it_done = t;
+        st = kStatusConverged;
+      }
+      if (!wave_any(running)) break;
+      apply_res<GS, 1>(L, res, running ? P.w6 : 0.0f);
+    } else if (stall) {
+      tot = half_sweep<GS, 1, 0, true>(L, lds, a, running ? P.w6 : 0.0f, res);
+    } else {
+      half_sweep<GS, 1, 0, false>(L, lds, a, running ? P.w6 : 0.0f, res);
+    }
+    store_colour<GS, 1>(L, lds, a, active);
+    __syncthreads();
+    const float w = running ? P.w6 : 0.0f;
+    if (stall || pre_check) {
+      const float s2 = half_sweep<GS, 0, 0, true>(L, lds, a, w, res);
+      sum_b = s2;
+      tot += s2;
+    } else {
+      half_sweep<GS, 0, 0, false>(L, lds, a, w, res);
+    }
+    store_colour<GS, 0>(L, lds, a, active);
+    __syncthreads();
+    if (stall) {
+      tot = seg_sum<SEG>(active ? tot : 0.0f);
+      if (running && t + 1 >= P.min_iter && tot > old) {
+        running = false;
+        it_done = t + 1;
+        st = kStatusConverged;
+      }
+      old = tot;
+      if (!wave_any(running)) break;
+    }
+  }
+}
+
+// Non-finite check over the owned cells of each patch, then the patch's iteration count and status.
+template <int GS, int SEG>
+__device__ __forceinline__ void report(const LaneCells<GS>& L, const LanePatch& lp, bool active, bool fell_back,
+                                       int it_done, int st, int slot, int pl) {
+  constexpr int R = SorGeo<GS>::R;
+  bool bad = false;
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+      for (int zi = 0; zi < R; zi++) bad |= !isfinite(L.phi[yy][xx][zi]);
+  const uint64_t bad_mask = __ballot(active && bad);
+  uint64_t seg_mask = ~0ull;
+  if constexpr (SEG < 64) seg_mask = ((1ull << SEG) - 1) << (slot * SEG);
+  if (bad_mask & seg_mask) st |= kStatusNonFinite;
+  if (fell_back) st |= kStatusFallback;
+  if (lp.valid && pl == 0) {
+    if (lp.iters) *lp.iters = it_done;
+    if (lp.status) *lp.status = st;
+  }
+}
+
+// Compact layout, phase A/B: the lane's Dirichlet values below and above its columns, its cells (interpolated
+// start or a caller's guess; zero otherwise), and the Dirichlet cells of the x and y faces in their slots.
+// The interpolation is the reference's per coarse cell (interp_children, all 8 children, the same operation
+// order), distributed by coarse cell: with U = 1 the lane's cells x = 2m+1+cx, y = 2yb+1+cy are the children
+// of coarse column (m+2, yb+2); coarse cells k = 1 and 10 give its z = 0 and 17 values, k = 2..9 its cells.
+// The 256 coarse cells under the four side faces (8 x 8 each) give the x and y Dirichlet cells, four per lane.
+// The coarse cube is staged in the (still unused) LDS buffer first; the face cells reach their slots after a
+// barrier, once every lane has read the staged cube.
+template <int GS>
+__device__ __forceinline__ void compact_start(const LanePatch& lp, float* __restrict__ lds, const SolveParams& P,
+                                              int m, int yb, int pl, bool active, LaneCells<GS>& L) {
+  using S = SorGeo<GS>;
+  constexpr int N = S::N, R = S::R, C = S::C;
+  static_assert(S::gs == 5 && S::U == 1 && R == 16, "compact layout: ghost size 5");
+  static_assert(C * C * C <= Compact::kFloats, "coarse staging fits the buffer");
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int xx = 0; xx < 2; xx++) {
+      L.bz[yy][xx][0] = L.bz[yy][xx][1] = 0.0f;
+#pragma unroll
+      for (int zi = 0; zi < R; zi++) L.phi[yy][xx][zi] = 0.0f;
+    }
+  constexpr int PER = 4 * R * R / 64;   // face cells per lane (whole-cube guess)
+  float face[PER];                      // this lane's face cells, published after the barrier
+  if (P.init_mode == 3) {
+    // the whole-cube guess brings the Dirichlet layer
+    if (active) {
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int xx = 0; xx < 2; xx++) {
+          const int col = (2 * yb + 1 + yy) * N + 2 * m + 1 + xx;
+          L.bz[yy][xx][0] = lp.guess[col];
+          L.bz[yy][xx][1] = lp.guess[(N - 1) * N * N + col];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int q = pl + 64 * k, f = q / (R * R), a = q % R, z = (q / R) % R + 1;
+      const int x = f < 2 ? (f == 0 ? 0 : N - 1) : a + 1, y = f < 2 ? a + 1 : (f == 2 ? 0 : N - 1);
+      face[k] = lp.valid ? lp.guess[(z * N + y) * N + x] : 0.0f;
+    }
+  } else {
+    // stage the coarse cube (C^3 floats, lane-contiguous loads, all issued before any store)
+    constexpr int CPL = (C * C * C + 63) / 64;
+    float cv[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; k++) {
+      const int t = pl + 64 * k;
+      cv[k] = lp.valid && t < C * C * C ? lp.cpot[t] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < CPL; k++) {
+      const int t = pl + 64 * k;
+      if (t < C * C * C) lds[t] = cv[k];
+    }
+    __syncthreads();
+    auto coarse = [&](int i, int j, int k, float (&v)[3][3][3]) {
+#pragma unroll
+      for (int dk = 0; dk < 3; dk++)
+#pragma unroll
+        for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+          for (int di = 0; di < 3; di++) v[dk][dj][di] = lds[((k + dk - 1) * C + (j + dj - 1)) * C + (i + di - 1)];
+    };
+    // the side faces: coarse cells (f, a, b) -> i or j = 1 / 10 and the other two 2..9; four children each
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = pl + 64 * u, f = q >> 6, a = (q & 7) + 2, b = ((q >> 3) & 7) + 2;
+      const int ci = f < 2 ? (f == 0 ? 1 : 10) : a, cj = f < 2 ? a : (f == 2 ? 1 : 10);
+      float v[3][3][3], ch[2][2][2];
+      coarse(ci, cj, b, v);
+      interp_children(v, P.int_scheme, ch);
+#pragma unroll
+      for (int c1 = 0; c1 < 2; c1++)
+#pragma unroll
+        for (int cz = 0; cz < 2; cz++)
+          face[4 * u + 2 * c1 + cz] = f < 2 ? ch[cz][c1][f == 0 ? 1 : 0] : ch[cz][f == 2 ? 1 : 0][c1];
+    }
+    // the lane's column: k = 1 and 10 always, k = 2..9 for the interpolated start
+    const int i = m + 2, j = yb + 2;
+#pragma unroll
+    for (int k = 1; k <= 10; k++) {
+      if (k != 1 && k != 10 && P.init_mode != 0) continue;
+      float v[3][3][3], ch[2][2][2];
+      coarse(i, j, k, v);
+      interp_children(v, P.int_scheme, ch);
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int xx = 0; xx < 2; xx++) {
+          if (k == 1) {
+            L.bz[yy][xx][0] = ch[1][yy][xx];
+          } else if (k == 10) {
+            L.bz[yy][xx][1] = ch[0][yy][xx];
+          } else {
+            L.phi[yy][xx][2 * k - 4] = ch[0][yy][xx];
+            L.phi[yy][xx][2 * k - 3] = ch[1][yy][xx];
+          }
+        }
+    }
+    __syncthreads();   // every lane has read the staged cube
+  }
+  // publish the face cells
+  if (lp.valid) {
+    if (P.init_mode == 3) {
+#pragma unroll
+      for (int k = 0; k < PER; k++) {
+        const int q = pl + 64 * k, f = q / (R * R), a = q % R, z = (q / R) % R + 1;
+        const int x = f < 2 ? (f == 0 ? 0 : N - 1) : a + 1, y = f < 2 ? a + 1 : (f == 2 ? 0 : N - 1);
+        lds[Compact::slot(x, y, z)] = face[k];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = pl + 64 * u, f = q >> 6, a = (q & 7) + 2, b = ((q >> 3) & 7) + 2;
+        const int ci = f < 2 ? (f == 0 ? 1 : 10) : a, cj = f < 2 ? a : (f == 2 ? 1 : 10);
+#pragma unroll
+        for (int c1 = 0; c1 < 2; c1++)
+#pragma unroll
+          for (int cz = 0; cz < 2; cz++) {
+            const int z = 2 * (b - 1) + cz - 1;
+            if (f < 2)   // x = 0 (child cx = 1) or x = 17 (cx = 0); c1 = cy
+              lds[Compact::slot(f == 0 ? 0 : N - 1, 2 * (cj - 1) + c1 - 1, z)] = face[4 * u + 2 * c1 + cz];
+            else         // y = 0 (cy = 1) or y = 17 (cy = 0); c1 = cx
+              lds[Compact::slot(2 * (ci - 1) + c1 - 1, f == 2 ? 0 : N - 1, z)] = face[4 * u + 2 * c1 + cz];
+          }
+      }
+    }
+  }
+  // owned density (and a caller's guess)
+  float2 r2[2][R], g2[2][R];
+  load_owned<GS>(lp, P, m, yb, active, r2, g2);
+#pragma unroll
+  for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+    for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+      for (int zi = 0; zi < R; zi++) {
+        if (P.init_mode >= 2) L.phi[yy][xx][zi] = xx ? g2[yy][zi].y : g2[yy][zi].x;
+        L.cr[yy][xx][zi] = active ? P.cst * (xx ? r2[yy][zi].y : r2[yy][zi].x) : 0.0f;
+      }
+}
+
+// Compact layout (ghost size 5, one patch per wavefront): the window needs gra_ghost <= 4, all of it interior.
+template <int GS, int PPW>
+__device__ __forceinline__ void sor_task_compact(float* __restrict__ lds, const LanePatch& lp, const SolveParams& P) {
+  using S = SorGeo<GS>;
+  constexpr int R = S::R, MX = S::MX;
+  static_assert(PPW == 1 && S::LP == 64, "compact layout: one ghost-size-5 patch per wavefront");
+  int lane = threadIdx.x;
+  asm volatile("" : "+v"(lane));
+  const int pl = lane;
+  const bool active = lp.valid;
+  const int m = pl / MX, yb = pl % MX;
+  CompactAddr<GS> a;
+  a.init(m, yb);
+  LaneCells<GS> L;
+  compact_start<GS>(lp, lds, P, m, yb, pl, active, L);
+  store_colour<GS, 0>(L, lds, a, active);   // the first sweep (colour 1) reads colour 0
+  __syncthreads();
+
+  float res[2 * R];
+  bool fell_back = false;
+  if (P.init_mode >= 2 && P.guess_tol > 0.0f) fell_back = guess_check<GS, 64, true>(L, lds, a, P, lp.valid, active, res);
+  int it_done, st;
+  sor_iterate<GS, 64>(L, lds, a, P, active, res, it_done, st);
+  report<GS, 64>(L, lp, active, fell_back, it_done, st, 0, pl);
+  __syncthreads();
+
+  // D. output window from a dense G^3 copy of the window (interior cells only, o = 5 - gra_ghost >= 1)
+  const int G = gra_nxt(P.gra_ghost), o = S::gs - P.gra_ghost;
+  if (active) {
+#pragma unroll
+    for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+      for (int xx = 0; xx < 2; xx++)
+#pragma unroll
+        for (int zi = 0; zi < R; zi++) {
+          const int x = 2 * m + 1 + xx - o, y = 2 * yb + 1 + yy - o, z = zi + 1 - o;
+          if ((unsigned)x < (unsigned)G && (unsigned)y < (unsigned)G && (unsigned)z < (unsigned)G)
+            lds[(z * G + y) * G + x] = L.phi[yy][xx][zi];
+        }
+  }
+  __syncthreads();
+  if (lp.valid) {
+    for (int t = pl; t < G * G * G; t += 64) lp.out[t] = lds[t];
+    if (P.fuse_gravity && lp.flu)
+      gravity_from_lds<GS, 64>(lds, [=](int x, int y, int z) { return ((z - o) * G + (y - o)) * G + (x - o); },
+                               lp.flu, pl, P.gra_const, P.p5);
+  }
+  __syncthreads();
+}
+
+// Solve up to PPW patches of ghost size GS with one wavefront (blockDim.x == 64).
+template <int GS, int PPW>
+__device__ __forceinline__ void sor_task(float* __restrict__ lds_wave, const LanePatch& lp, const SolveParams& P) {
+  using S = SorGeo<GS>;
+  if constexpr (S::COMPACT) {
+    sor_task_compact<GS, PPW>(lds_wave, lp, P);
+  } else {
+  constexpr int N = S::N, R = S::R, HP = S::HP, MX = S::MX, LP = S::LP;
+  constexpr int SEG = 64 / PPW;
+  static_assert(LP <= SEG, "patch does not fit its lane segment");
+  int lane = threadIdx.x;
+  // opaque per patch: inside the persistent kernel's task loop the compiler would otherwise hoist every
+  // lane-derived address out of the loop and keep them all live (measured: 2 KB of spills per lane)
+  asm volatile("" : "+v"(lane));
+  const int slot = lane / SEG, pl = lane % SEG;
+  float* lds = lds_wave + slot * S::FLOATS;
+  const bool active = lp.valid && pl < LP;
+  const int plc = pl < LP ? pl : LP - 1;  // idle lanes mirror a real lane's addresses and never write
+  const int m = S::XMAJOR ? plc / MX : plc % MX, yb = S::XMAJOR ? plc % MX : plc / MX;
+  const int lb = 2 * yb * HP + m;
+  const SplitAddr<GS> a{lb};
+
+  // A. coarse -> fine interpolation of the whole cube (initial guess and Dirichlet layer); a zero start or a
+  // caller's guess needs only the Dirichlet layer (a rejected guess restarts from zero, not from the
+  // interpolation), and a whole-cube guess brings its own Dirichlet layer
+  if (lp.valid) {
+    if (P.init_mode == 3)
+      shell_to_lds<GS, SEG>(lp.guess, lds, pl);
+    else if (P.init_mode != 0)
+      interp_to_lds<GS, true>(lp.cpot, lds, pl, SEG, P.int_scheme);
+    else
+      interp_to_lds<GS, false>(lp.cpot, lds, pl, SEG, P.int_scheme);
+  }
+  __syncthreads();
+
+  // B. owned cells into registers
+  LaneCells<GS> L;
+  float2 r2[2][R], g2[2][R];
+  load_owned<GS>(lp, P, m, yb, active, r2, g2);
 #pragma unroll
   for (int yy = 0; yy < 2; yy++) {
 #pragma unroll
@@ -446,111 +857,13 @@ __device__ __forceinline__ void sor_task(float* __restrict__ lds_wave, const Lan
   }
   __syncthreads();
 
-  // B'. reject a caller guess whose relative residual exceeds guess_tol: restart from the zero guess
+  // B'. guess check, C. iterations
   float res[2 * R];
   bool fell_back = false;
-  if (P.init_mode >= 2 && P.guess_tol > 0.0f) {
-    float num = half_sweep<GS, 1, 1, true>(L, lds, lb, 0.0f, res);
-    num += half_sweep<GS, 0, 1, true>(L, lds, lb, 0.0f, res);
-    float den = 0.0f;
-#pragma unroll
-    for (int yy = 0; yy < 2; yy++)
-#pragma unroll
-      for (int xx = 0; xx < 2; xx++)
-#pragma unroll
-        for (int zi = 0; zi < R; zi++) den += fabsf(L.phi[yy][xx][zi]);
-    if (!active) num = den = 0.0f;
-    num = seg_sum<SEG>(num);
-    den = seg_sum<SEG>(den);
-    fell_back = lp.valid && !(num <= P.guess_tol * den);
-    if (fell_back) {
-#pragma unroll
-      for (int yy = 0; yy < 2; yy++)
-#pragma unroll
-        for (int xx = 0; xx < 2; xx++)
-#pragma unroll
-          for (int zi = 0; zi < R; zi++) L.phi[yy][xx][zi] = 0.0f;
-    }
-    store_colour<GS, 1>(L, lds, lb, active && fell_back);
-    store_colour<GS, 0>(L, lds, lb, active && fell_back);
-    __syncthreads();
-  }
-
-  // C. red-black iterations
-  bool running = active;
-  int it_done = P.max_iter, st = kStatusMaxIter;
-  const bool relres = P.term_mode == 1, stall = P.term_mode == 0;
-  const int first_check = P.min_iter > 1 ? P.min_iter : 1;
-  float sum_b = 0.0f, old = FLT_MAX;
-  for (int t = 0; t < P.max_iter; ++t) {
-    const bool check = relres && t >= first_check && (t - first_check) % P.check_every == 0;
-    const bool pre_check = relres && t + 1 >= first_check && (t + 1 - first_check) % P.check_every == 0;
-    float tot = 0.0f;
-    if (check) {
-      float num = half_sweep<GS, 1, 1, true>(L, lds, lb, 0.0f, res), den = 0.0f;
-#pragma unroll
-      for (int yy = 0; yy < 2; yy++)
-#pragma unroll
-        for (int xx = 0; xx < 2; xx++)
-#pragma unroll
-          for (int zi = 0; zi < R; zi++) den += fabsf(L.phi[yy][xx][zi]);
-      num = num + P.damp * sum_b;
-      if (!active) num = den = 0.0f;
-      num = seg_sum<SEG>(num);
-      den = seg_sum<SEG>(den);
-      if (running && num <= P.rel_tol * den) {
-        running = false;
-        it_done = t;
-        st = kStatusConverged;
-      }
-      if (!wave_any(running)) break;
-      apply_res<GS, 1>(L, res, running ? P.w6 : 0.0f);
-    } else if (stall) {
-      tot = half_sweep<GS, 1, 0, true>(L, lds, lb, running ? P.w6 : 0.0f, res);
-    } else {
-      half_sweep<GS, 1, 0, false>(L, lds, lb, running ? P.w6 : 0.0f, res);
-    }
-    store_colour<GS, 1>(L, lds, lb, active);
-    __syncthreads();
-    const float w = running ? P.w6 : 0.0f;
-    if (stall || pre_check) {
-      const float s2 = half_sweep<GS, 0, 0, true>(L, lds, lb, w, res);
-      sum_b = s2;
-      tot += s2;
-    } else {
-      half_sweep<GS, 0, 0, false>(L, lds, lb, w, res);
-    }
-    store_colour<GS, 0>(L, lds, lb, active);
-    __syncthreads();
-    if (stall) {
-      tot = seg_sum<SEG>(active ? tot : 0.0f);
-      if (running && t + 1 >= P.min_iter && tot > old) {
-        running = false;
-        it_done = t + 1;
-        st = kStatusConverged;
-      }
-      old = tot;
-      if (!wave_any(running)) break;
-    }
-  }
-
-  // non-finite check over the owned cells of each patch
-  bool bad = false;
-#pragma unroll
-  for (int yy = 0; yy < 2; yy++)
-#pragma unroll
-    for (int xx = 0; xx < 2; xx++)
-#pragma unroll
-      for (int zi = 0; zi < R; zi++) bad |= !isfinite(L.phi[yy][xx][zi]);
-  const uint64_t bad_mask = __ballot(active && bad);
-  uint64_t seg_mask = ~0ull;
-  if constexpr (SEG < 64) seg_mask = ((1ull << SEG) - 1) << (slot * SEG);
-  if (bad_mask & seg_mask) st |= kStatusNonFinite;
-  if (fell_back) st |= kStatusFallback;
-  if (lp.valid && pl == 0) {
-    if (lp.iters) *lp.iters = it_done;
-    if (lp.status) *lp.status = st;
-  }
+  if (P.init_mode >= 2 && P.guess_tol > 0.0f) fell_back = guess_check<GS, SEG, false>(L, lds, a, P, lp.valid, active, res);
+  int it_done, st;
+  sor_iterate<GS, SEG>(L, lds, a, P, active, res, it_done, st);
+  report<GS, SEG>(L, lp, active, fell_back, it_done, st, slot, pl);
   __syncthreads();
 
   // D. output window: out[k][j][i] = cube[k+o][j+o][i+o], o = GS - GRA_GHOST_SIZE
@@ -560,9 +873,12 @@ __device__ __forceinline__ void sor_task(float* __restrict__ lds_wave, const Lan
       const int i = t % G, j = (t / G) % G, k = t / (G * G);
       lp.out[t] = lds[cube_index(N, HP, i + o, j + o, k + o)];
     }
-    if (P.fuse_gravity && lp.flu) gravity_from_lds<GS, SEG>(lds, lp.flu, pl, P.gra_const, P.p5);
+    if (P.fuse_gravity && lp.flu)
+      gravity_from_lds<GS, SEG>(lds, [](int x, int y, int z) { return cube_index(N, HP, x, y, z); }, lp.flu, pl,
+                                P.gra_const, P.p5);
   }
   __syncthreads();
+  }
 }
 
 // ---- kernels -----------------------------------------------------------------------------------------
@@ -701,6 +1017,10 @@ static hipError_t launch_sor_tasks_as(const MixedBatch& b, const SolveParams& p,
 template <int GS, int PPW>
 static hipError_t launch_sor_tasks_pad(const MixedBatch& b, const SolveParams& p, int pad, int first, int count,
                                        hipStream_t s) {
+  if constexpr (GS == 5 && PPW == 1) {
+    if (pad == kPadCompact) return p.gra_ghost <= 4 ? launch_sor_tasks_as<5, 1, kPadCompact>(b, p, first, count, s)
+                                                    : launch_sor_tasks_as<5, 1, 1>(b, p, first, count, s);
+  }
   switch (pad) {
     case 0: return launch_sor_tasks_as<GS, PPW, 0>(b, p, first, count, s);
     case 1: return launch_sor_tasks_as<GS, PPW, 1>(b, p, first, count, s);
@@ -785,14 +1105,22 @@ int default_ppw(int gs) {
   return 1;
 }
 
-// Measured on MI355X (16384 patches, 1e-5): ghost size 4 gains 17% with 2 floats of row padding (HP = 10
-// breaks the 16-float row stride that puts every other lane row on the same banks); the other ghost sizes
-// are best unpadded, where padding costs a resident patch per CU.
-int default_pad(int gs) { return gs == 4 ? 2 : 0; }
+// Measured on MI355X (16384 patches, 1e-5, plain and interpolated starts, every padding 0-3): one float of
+// row padding is fastest at ghost sizes 1, 4 and 5 (at 5 it turns the x-major lane order conflict-free, +7%
+// over the unpadded cube although a CU then holds 6 patches instead of 7; at 1 it gains 9%); ghost sizes 2
+// and 3 are best unpadded.
+int default_pad(int gs) { return gs == 5 ? kPadCompact : gs == 1 || gs == 4 ? 1 : 0; }
 
 size_t lds_bytes_sor(int gs, int ppw, int pad) {
   const Geo g = geo_of(gs);
+  if (pad == kPadCompact) return sizeof(float) * (size_t)ppw * Compact::kFloats;
   return sizeof(float) * (size_t)ppw * 2 * g.N * g.N * (g.N / 2 + pad);
+}
+
+bool supported_layout(int gs, int ppw, int pad) {
+  if (!supported_ppw(gs, ppw)) return false;
+  if (pad == kPadCompact) return gs == 5 && ppw == 1;
+  return pad >= 0 && pad <= 3;
 }
 
 template <int GS, int PPW, int PAD>
@@ -806,6 +1134,12 @@ static hipError_t launch_sor_uniform(const UniformBatch& b, const SolveParams& p
 
 template <int GS, int PPW>
 static hipError_t launch_sor_pad(const UniformBatch& b, const SolveParams& p, int pad, hipStream_t s) {
+  if constexpr (GS == 5 && PPW == 1) {
+    // the compact layout's window holds interior cells only: a window reaching the Dirichlet layer
+    // (gra_ghost 5) runs in the split layout
+    if (pad == kPadCompact) return p.gra_ghost <= 4 ? launch_sor_uniform<5, 1, kPadCompact>(b, p, s)
+                                                    : launch_sor_uniform<5, 1, 1>(b, p, s);
+  }
   switch (pad) {
     case 0: return launch_sor_uniform<GS, PPW, 0>(b, p, s);
     case 1: return launch_sor_uniform<GS, PPW, 1>(b, p, s);
@@ -839,6 +1173,9 @@ static int sor_blocks_as() {
 
 template <int GS, int PPW>
 static int sor_blocks_pad(int pad) {
+  if constexpr (GS == 5 && PPW == 1) {
+    if (pad == kPadCompact) return sor_blocks_as<5, 1, kPadCompact>();
+  }
   switch (pad) {
     case 0: return sor_blocks_as<GS, PPW, 0>();
     case 1: return sor_blocks_as<GS, PPW, 1>();

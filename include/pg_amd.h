@@ -180,9 +180,13 @@ int pgamd_scatter_patch_groups_device(int gra_ghost_size, int ngroup, const int*
                                       float* d_fine_flu, void* stream);
 
 /* Launch layout of the SOR kernel for one ghost size: patches per wavefront (1, 2 or 4 where they fit) and
- * the padding of the colour-split LDS rows (0..3 floats; trades LDS capacity for fewer bank conflicts).
- * This is the autotuner's search space (pgamd_autotune); no reference counterpart: the reference fixes its
- * launch shapes per GPU generation with macros (Header/CUPOT.h:38-152). */
+ * the padding of the colour-split LDS rows (0..3 floats; trades LDS capacity for fewer bank conflicts), or
+ * PGAMD_LAYOUT_COMPACT at ghost size 5 with one patch per wavefront: a single-buffer LDS cube of 16 KB instead
+ * of 25.9 KB, eight resident patches per CU instead of six (the default there; a solve whose gra_ghost_size
+ * is 5 runs that ghost size in the split layout with one float of padding).  Results do not depend on the
+ * layout.  This is the autotuner's search space (pgamd_autotune); no reference counterpart: the reference
+ * fixes its launch shapes per GPU generation with macros (Header/CUPOT.h:38-152). */
+#define PGAMD_LAYOUT_COMPACT 4
 int pgamd_set_layout(pgamd_context* ctx, int ghost_size, int patches_per_wave, int row_pad);
 int pgamd_get_layout(pgamd_context* ctx, int ghost_size, int* patches_per_wave, int* row_pad);
 /* Residency of the SOR kernel at the context's layout for one ghost size: workgroups (one wavefront each) the

@@ -328,10 +328,73 @@ def test_full_cube_output_window(scheme):
             ctx.solve_device(rho_d, cpot_d, out_d, flu_d, 8, pg.solver_args(graacc=True))
 
 
+@pytest.mark.parametrize("init_mode,term_mode,gg,fuse,tol", [
+    (pg.INIT_INTERP, pg.TERM_REFERENCE, 1, True, 0.0),
+    (pg.INIT_ZERO, pg.TERM_RELRES, 1, True, 0.0),
+    (pg.INIT_INTERP, pg.TERM_RELRES, 2, True, 0.0),
+    (pg.INIT_ZERO, pg.TERM_REFERENCE, 4, False, 0.0),
+    (pg.INIT_ZERO, pg.TERM_REFERENCE, 5, False, 0.0),          # whole-cube window: runs in the split layout
+    (pg.INIT_GUESS, pg.TERM_RELRES, 1, True, 1e-2),            # guess check, bad guesses rejected
+    (pg.INIT_GUESS_CUBE, pg.TERM_RELRES, 1, True, 0.0),        # no guess check
+    (pg.INIT_GUESS_CUBE, pg.TERM_RELRES, 1, True, 1e30),       # guess check, every guess kept
+    (pg.INIT_GUESS_CUBE, pg.TERM_REFERENCE, 1, True, 1e-2),    # guess check, every guess rejected
+])
+def test_compact_layout_matches_split(init_mode, term_mode, gg, fuse, tol):
+    """The compact single-buffer LDS cube of ghost size 5 (PGAMD_LAYOUT_COMPACT) and the colour-split cube give
+    bitwise the same potential window, fluid, iteration counts and status flags, for every start (the compact
+    kernel interpolates by coarse column and face, the split kernel by coarse cell), both stopping rules, output
+    windows 1..5, the guess check kept, rejected or off, and both interpolation schemes."""
+    import torch
+
+    gs, n = 5, 192
+    rho, cpot, flu = _mixed_inputs(n, gs, seed=11)
+    g = pg.geometry(gs, gg)
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(5)
+    gshape = (n, g["N"], g["N"], g["N"]) if init_mode == pg.INIT_GUESS_CUBE else (n, g["R"], g["R"], g["R"])
+    guess = None
+    if init_mode in (pg.INIT_GUESS, pg.INIT_GUESS_CUBE):
+        # half the patches get a plausible guess (a scaled interpolated cube), half a zero guess
+        cube = O.interp(cpot, gs, pg.INT_CQUAD, dtype=np.float32)
+        guess = cube if init_mode == pg.INIT_GUESS_CUBE else np.ascontiguousarray(cube[:, 1:-1, 1:-1, 1:-1])
+        guess = (guess * rng.uniform(0.9, 1.1, size=(n, 1, 1, 1))).astype(np.float32)
+        guess[::2] = 0.0
+        assert guess.shape == gshape
+    results = {}
+    for ints in (pg.INT_CQUAD, pg.INT_QUAD):
+        args = pg.solver_args(dt=0.5, sor_min_iter=10, sor_max_iter=300, sor_omega=1.8, poi_coeff=POI,
+                              int_scheme=ints, graacc=fuse)
+        for lay in (1, pg.LAYOUT_COMPACT):
+            print(f"[case] compact vs split: init={init_mode} term={term_mode} gg={gg} fuse={fuse} tol={tol} "
+                  f"ints={ints} layout pad {lay}", flush=True)
+            with pg.Context(0) as ctx:
+                ctx.set_options(pot_ghost_size=gs, gra_ghost_size=gg, init_mode=init_mode, term_mode=term_mode,
+                                check_every=2, rel_tol=1e-5, fuse_gravity=int(fuse), guess_fallback_tol=tol)
+                ctx.set_layout(gs, 1, lay)
+                d_rho = torch.from_numpy(rho).to(dev)
+                d_cpot = torch.from_numpy(cpot).to(dev)
+                d_flu = torch.from_numpy(flu).to(dev)
+                d_out = torch.zeros((n, g["G"], g["G"], g["G"]), device=dev)
+                d_guess = torch.from_numpy(guess).to(dev) if guess is not None else None
+                d_it = torch.zeros(n, dtype=torch.int32, device=dev)
+                d_st = torch.zeros(n, dtype=torch.int32, device=dev)
+                ctx.solve_device(d_rho, d_cpot, d_out, d_flu, n, args, guess=d_guess, iters=d_it, status=d_st)
+                torch.cuda.synchronize()
+                results[(ints, lay)] = [t.cpu().numpy() for t in (d_out, d_flu, d_it, d_st)]
+        for a, b in zip(results[(ints, 1)], results[(ints, pg.LAYOUT_COMPACT)]):
+            assert np.array_equal(a, b)
+    st = results[(pg.INT_CQUAD, pg.LAYOUT_COMPACT)][3]
+    if tol == 1e-2:
+        assert ((st & pg.STATUS_FALLBACK) != 0).sum() >= n // 2      # every zero guess is rejected
+    if tol == 1e30:   # only the zero guesses (relative residual infinite) are rejected
+        assert ((st & pg.STATUS_FALLBACK) != 0).sum() == n // 2
+
+
 def test_layout_occupancy():
     """pgamd_layout_occupancy: resident SOR workgroups per CU at every ghost size and layout, consistent with the
     layout's patches per wavefront; at ghost size 5 and one patch per wavefront, LDS (the colour-split cube) and
-    registers allow between 4 and 8 resident workgroups."""
+    registers allow between 4 and 8 resident workgroups, and the compact cube (16 KB) reaches the register
+    limit of 8.  The compact layout exists only for ghost size 5 with one patch per wavefront."""
     with pg.Context(0) as ctx:
         for gs in range(1, 6):
             ppw, pad = ctx.get_layout(gs)
@@ -342,5 +405,12 @@ def test_layout_occupancy():
         ctx.set_layout(5, 1, 0)
         wg, patches = ctx.layout_occupancy(5)
         assert wg == patches and 4 <= wg <= 8
+        ctx.set_layout(5, 1, pg.LAYOUT_COMPACT)
+        wg, patches = ctx.layout_occupancy(5)
+        print(f"[occupancy] gs=5 compact layout: {wg} workgroups per CU", flush=True)
+        assert wg == patches == 8
+        for gs, ppw in ((4, 1), (3, 1), (1, 4)):
+            with pytest.raises(pg.PgError):
+                ctx.set_layout(gs, ppw, pg.LAYOUT_COMPACT)
         with pytest.raises(pg.PgError):
             ctx.layout_occupancy(6)
