@@ -51,7 +51,7 @@ struct SorGeo {
   static constexpr int MX = R / 2;              // x-pairs per row
   static constexpr int LP = MX * (R / 2);       // lanes per patch
   static constexpr bool COMPACT = PAD == kPadCompact;   // single-buffer layout (ghost size 5 only)
-  static constexpr int FLOATS = COMPACT ? 16 * 256 : 2 * N * N * HP;   // LDS floats per patch
+  static constexpr int FLOATS = COMPACT ? 16 * 256 + 16 * 64 : 2 * N * N * HP;   // LDS floats per patch
   // Lane order within a patch.  Row-major (lane = yb*MX + m) puts consecutive lanes on consecutive words of a
   // split row; x-major (lane = m*MX + yb) puts them 2*HP words apart.  An LDS access serves 32 lanes per
   // cycle over 32 four-byte banks, so x-major at HP = 10 (ghost size 5, one float of padding) sends the 32
@@ -238,6 +238,11 @@ struct SplitAddr {
     using H = HalfSweep<GS, COL>;
     return lb + own_off<GS>(COL, H::zi(q) + 1, H::yy(q), H::xx(q));
   }
+  template <int COL>
+  __device__ __forceinline__ float crv(const LaneCells<GS>& L, const float* __restrict__, int q) const {
+    using H = HalfSweep<GS, COL>;
+    return L.cr[H::yy(q)][H::xx(q)][H::zi(q)];
+  }
 };
 
 // Compact layout (ghost size 5, one patch per wavefront).  A red-black half-sweep reads only the other
@@ -247,13 +252,18 @@ struct SplitAddr {
 // interior row y (1..16) at (y-1)*kRS with slot 0 the Dirichlet cell x = 0, slots 1..8 the pairs and slot 9
 // the Dirichlet cell x = 17; the Dirichlet rows y = 0 and 17 keep their 16 interior cells (x = 2m+1 at A+m,
 // x = 2m+2 at B+m).  The z = 0 and 17 planes stay in the lanes' registers.  16 KB per patch instead of the
-// split layout's 25.9 KB: eight resident patches per CU instead of six.
+// split layout's 25.9 KB: eight resident patches per CU instead of six.  The 4 KB after the cube hold the
+// lane's Const*rho of its top four planes ([cell][lane], conflict-free): with them out of the registers the
+// sweep fits 256 VGPRs without spilling, and eight 20 KB patches fill the 160 KB of LDS exactly.
 // Banks: the 32 lanes of an LDS cycle (yb = 0..7, four values of m) read rows 20*yb + m + const, 28 of the 32
 // four-byte banks for a y-neighbour read; A and B sit on the 4 banks left free (23..26 mod 32 for y = 0, 1..4
 // for y = 17).  kPS = 256 is a multiple of 64 floats, so two planes pair into one ds_read2st64 from the
 // lane's base: the sweeps need no address arithmetic.
 struct Compact {
-  static constexpr int kRS = 10, kPS = 256, kPlanes = 16, kFloats = kPlanes * kPS;
+  static constexpr int kRS = 10, kPS = 256, kPlanes = 16;
+  static constexpr int kCr = kPlanes * kPS;   // Const*rho of the lane's cells at z - 1 >= kCrZ, 16 per lane
+  static constexpr int kCrZ = 12;
+  static constexpr int kFloats = kCr + 16 * 64;
   static constexpr int kY0A = 183, kY0B = 215, kY17A = 161, kY17B = 193;
   // slot of cube cell (x, y, z), x, y, z in 1..16 plus the Dirichlet cells x or y in {0, 17}, z in 1..16
   __device__ static constexpr int slot(int x, int y, int z) {
@@ -272,7 +282,10 @@ struct CompactAddr {
   int b;          // 20*yb + m
   int ylo[2];     // y-neighbour below (yy = 0) of the lane's cell xx, plane 1
   int yhi[2];     // y-neighbour above (yy = 1)
-  __device__ __forceinline__ void init(int m, int yb) {
+  int crb;        // kCr + lane
+  __device__ __forceinline__ void init(int m, int yb, int lane) {
+    crb = Compact::kCr + lane;
+    asm volatile("" : "+v"(crb));
     b = 2 * yb * Compact::kRS + m;
 #pragma unroll
     for (int xx = 0; xx < 2; xx++) {
@@ -294,12 +307,22 @@ struct CompactAddr {
     using H = HalfSweep<GS, COL>;
     return b + Compact::kRS * H::yy(q) + 1 + Compact::kPS * H::zi(q);
   }
+  // LDS slot of Const*rho of the lane's cell (yy, xx, zi), zi >= kCrZ, relative to crb
+  __device__ __forceinline__ static constexpr int cr_slot(int yy, int xx, int zi) {
+    return (((zi - Compact::kCrZ) * 2 + yy) * 2 + xx) * 64;
+  }
+  template <int COL>
+  __device__ __forceinline__ float crv(const LaneCells<GS>& L, const float* __restrict__ lds, int q) const {
+    using H = HalfSweep<GS, COL>;
+    const int yy = H::yy(q), zi = H::zi(q), xx = H::xx(q);
+    return zi >= Compact::kCrZ ? lds[crb + cr_slot(yy, xx, zi)] : L.cr[yy][xx][zi];
+  }
 };
 
 // Pre-update residual of owned cell q of colour COL, reference operand order
 // (CPU_PoissonSolver_SOR.cpp:247-250): ((((((kp+km)+jp)+jm)+ip)+im) - 6 phi) - Const*rho.
 template <int GS, int COL>
-__device__ __forceinline__ float residual_at(const LaneCells<GS>& L, const float (&v)[2], int q) {
+__device__ __forceinline__ float residual_at(const LaneCells<GS>& L, const float (&v)[2], int q, float cr) {
   using H = HalfSweep<GS, COL>;
   constexpr int R = H::R;
   const int yy = H::yy(q), zi = H::zi(q), xx = H::xx(q);
@@ -309,7 +332,7 @@ __device__ __forceinline__ float residual_at(const LaneCells<GS>& L, const float
   const float jm = yy == 1 ? L.phi[0][xx][zi] : v[0];
   const float ip = xx == 0 ? L.phi[yy][1][zi] : v[1];
   const float im = xx == 1 ? L.phi[yy][0][zi] : v[1];
-  return kp + km + jp + jm + ip + im - 6.0f * L.phi[yy][xx][zi] - L.cr[yy][xx][zi];
+  return kp + km + jp + jm + ip + im - 6.0f * L.phi[yy][xx][zi] - cr;
 }
 
 // Publish the lane's cells of colour COL to LDS (one exec-masked block per half-sweep).
@@ -339,7 +362,7 @@ __device__ __forceinline__ float half_sweep(LaneCells<GS>& L, const float* __res
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     if (q + LA < NQ) a.template load_nb<COL>(lds, q + LA, nb[q + LA]);
-    const float r = residual_at<GS, COL>(L, nb[q], q);
+    const float r = residual_at<GS, COL>(L, nb[q], q, a.template crv<COL>(L, lds, q));
     if (ACC) acc += fabsf(r);
     if (MODE == 0) {
       float& ph = L.phi[H::yy(q)][H::xx(q)][H::zi(q)];
@@ -738,6 +761,7 @@ __device__ __forceinline__ void compact_start(const LanePatch& lp, float* __rest
       for (int zi = 0; zi < R; zi++) {
         if (P.init_mode >= 2) L.phi[yy][xx][zi] = xx ? g2[yy][zi].y : g2[yy][zi].x;
         L.cr[yy][xx][zi] = active ? P.cst * (xx ? r2[yy][zi].y : r2[yy][zi].x) : 0.0f;
+        if (zi >= Compact::kCrZ) lds[Compact::kCr + CompactAddr<GS>::cr_slot(yy, xx, zi) + pl] = L.cr[yy][xx][zi];
       }
 }
 
@@ -753,7 +777,7 @@ __device__ __forceinline__ void sor_task_compact(float* __restrict__ lds, const 
   const bool active = lp.valid;
   const int m = pl / MX, yb = pl % MX;
   CompactAddr<GS> a;
-  a.init(m, yb);
+  a.init(m, yb, pl);
   LaneCells<GS> L;
   compact_start<GS>(lp, lds, P, m, yb, pl, active, L);
   store_colour<GS, 0>(L, lds, a, active);   // the first sweep (colour 1) reads colour 0
