@@ -736,7 +736,7 @@ __device__ __forceinline__ float interp_cube_lds(const float* cs, int int_scheme
   return mx;
 }
 
-template <int GS, bool CUBE>
+template <int GS, bool CUBE, bool KEEP>
 __device__ void fused_patch(const float* __restrict__ rho, const float* __restrict__ cpot, const u16* __restrict__ Wb,
                             const float* __restrict__ P, float cst, int int_scheme, float* __restrict__ guess,
                             FusedLds& L) {
@@ -792,8 +792,9 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   for (int fz = 0; fz < OFF; fz++) word(fz) = 0u;
 #pragma unroll
   for (int fz = OFF + R; fz < 16; fz++) word(fz) = 0u;
-  // The column's interpolated values are kept for the guess: in registers for the whole-cube guess (164 VGPRs),
-  // parked in the guess array otherwise (the interior and mixed-batch instances would spill).
+  // The column's interpolated values are kept for the guess: in registers in the uniform-batch instances (KEEP;
+  // 164 VGPRs for the whole-cube guess, 168 for the interior guess), parked in the guess array in the
+  // mixed-batch instance, whose five ghost sizes would spill.
   float icol[R];
   float* gcol = guess + (CUBE ? N * N + (y + 1) * N + (x + 1) : y * R + x);   // interior column, z = 0
   if (colv) {
@@ -805,7 +806,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
       const float cp = cz[N * N];
       const float r = cp + cm + cz[N] + cz[-N] + cz[1] + cz[-1] - 6.0f * c - cst * rcol[z];
       word(z + OFF) = pack2_bf16(c * inv, (r * inv) * kResScale);
-      if (CUBE)
+      if (KEEP)
         icol[z] = c;
       else
         gcol[z * R * R] = c;
@@ -848,23 +849,22 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
     const float* sd = reinterpret_cast<const float*>(L.X) + y * R + x;
 #pragma unroll
     for (int z = 0; z < R; z++) {
-      if (CUBE)
-        gcol[z * N * N] = icol[z] + sd[z * R * R];
-      else
-        gcol[z * R * R] = gcol[z * R * R] + sd[z * R * R];
+      constexpr int ZS = CUBE ? N * N : R * R;
+      const float base = KEEP ? icol[z] : gcol[z * ZS];
+      gcol[z * ZS] = base + sd[z * R * R];
     }
   }
 }
 
-template <bool CUBE>
+template <bool CUBE, bool KEEP>
 __device__ __forceinline__ void fused_dispatch(int gs, const float* r, const float* c, const u16* Wb, const float* P,
                                                float cst, int int_scheme, float* out, FusedLds& L) {
   switch (gs) {
-    case 1: fused_patch<1, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 2: fused_patch<2, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 3: fused_patch<3, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 4: fused_patch<4, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    default: fused_patch<5, CUBE>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 1: fused_patch<1, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 2: fused_patch<2, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 3: fused_patch<3, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 4: fused_patch<4, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    default: fused_patch<5, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
   }
 }
 
@@ -882,7 +882,7 @@ __global__ void __launch_bounds__(kFT, kFusedBlocksPerCU) fused_guess_kernel(con
   const float* r = rho + (size_t)p * g.R * g.R * g.R;
   const float* c = cpot + (size_t)p * g.C * g.C * g.C;
   float* out = guess + (size_t)p * (CUBE ? g.N * g.N * g.N : g.R * g.R * g.R);
-  fused_dispatch<CUBE>(gs, r, c, Wb, P, cst, int_scheme, out, L);
+  fused_dispatch<CUBE, true>(gs, r, c, Wb, P, cst, int_scheme, out, L);
 }
 
 // Mixed ghost sizes: patch p reads its ghost size and packed offsets from the plan's descriptors; the
@@ -897,7 +897,7 @@ __global__ void __launch_bounds__(kFT, kFusedBlocksPerCU) fused_guess_mixed_kern
   const int p = blockIdx.x;
   if (p >= npatch) return;
   const PatchDesc d = desc[p];
-  fused_dispatch<false>(d.gs, rho + d.rho_off, cpot + d.cpot_off, Wb, P, cst, int_scheme, guess + d.rho_off, L);
+  fused_dispatch<false, false>(d.gs, rho + d.rho_off, cpot + d.cpot_off, Wb, P, cst, int_scheme, guess + d.rho_off, L);
 }
 
 }  // namespace cnn
