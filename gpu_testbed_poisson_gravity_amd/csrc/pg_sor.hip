@@ -75,6 +75,29 @@ __device__ __forceinline__ float seg_sum(float v) {
   return v;
 }
 
+// Sum for the relative-residual test.  A whole wavefront (SEG = 64) reduces inside the VALU: DPP exchanges
+// within each row of 16 lanes (partners 1 and 2 apart in a quad, then the mirrored halves of 8 and of 16
+// lanes; each step adds the same two operands in every lane of a pair, only commuted), then the four row sums
+// read from lanes 0, 16, 32 and 48 are added as (r0 + r1) + (r2 + r3) from scalar registers, so every lane gets
+// the same bits.  seg_sum's butterfly crosses lanes through the LDS crossbar six times in a row; the stall
+// rule and the guess check keep that order.
+template <int SEG>
+__device__ __forceinline__ float check_sum(float v) {
+  if constexpr (SEG != 64) {
+    return seg_sum<SEG>(v);
+  } else {
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad [1,0,3,2]
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad [2,3,0,1]
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false)); // half-row mirror
+    v = v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false)); // row mirror
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (r0 + r1) + (r2 + r3);
+  }
+}
+
 // Any lane of the wavefront (a workgroup is exactly one wavefront here).
 __device__ __forceinline__ bool wave_any(bool pred) { return __ballot(pred) != 0; }
 
@@ -557,8 +580,8 @@ __device__ __forceinline__ void sor_iterate(LaneCells<GS>& L, float* __restrict_
           for (int zi = 0; zi < R; zi++) den += fabsf(L.phi[yy][xx][zi]);
       num = num + P.damp * sum_b;
       if (!active) num = den = 0.0f;
-      num = seg_sum<SEG>(num);
-      den = seg_sum<SEG>(den);
+      num = check_sum<SEG>(num);
+      den = check_sum<SEG>(den);
       if (running && num <= P.rel_tol * den) {
         running = false;
         it_done = t;

@@ -390,6 +390,42 @@ def test_compact_layout_matches_split(init_mode, term_mode, gg, fuse, tol):
         assert ((st & pg.STATUS_FALLBACK) != 0).sum() == n // 2
 
 
+def test_autotune_ghost_size_5_considers_the_compact_layout(tmp_path):
+    """The autotuner's search at ghost size 5 includes the compact cube (PGAMD_LAYOUT_COMPACT); whichever layout
+    it keeps, the solve is bitwise the default layout's, and a saved table with the compact layout loads back."""
+    import torch
+
+    gs, n = 5, 512
+    rho, cpot, flu, _ = I.plummer(n, gs, seed=21)
+    dev = torch.device("cuda", 0)
+    d_rho, d_cpot = torch.from_numpy(rho).to(dev), torch.from_numpy(cpot).to(dev)
+    g = pg.geometry(gs)
+    args = pg.solver_args(sor_min_iter=20, sor_max_iter=20, poi_coeff=POI, graacc=False)
+
+    def solve(ctx):
+        out = torch.zeros((n, g["G"], g["G"], g["G"]), device=dev)
+        ctx.solve_device(d_rho, d_cpot, out, None, n, args)
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    with pg.Context(0) as ctx:
+        ctx.set_options(pot_ghost_size=gs, init_mode=pg.INIT_INTERP, fuse_gravity=0)
+        assert ctx.get_layout(gs) == (1, pg.LAYOUT_COMPACT)      # the default at ghost size 5
+        base = solve(ctx)
+        out = torch.zeros((n, g["G"], g["G"], g["G"]), device=dev)
+        print("[case] autotune gs=5", flush=True)
+        ppw, pad, ms = ctx.autotune(gs, d_rho, d_cpot, out, n, args, repeats=2)
+        assert ms > 0 and ppw == 1 and pad in (0, 1, 2, 3, pg.LAYOUT_COMPACT)
+        assert np.array_equal(solve(ctx), base)
+        ctx.set_layout(gs, 1, pg.LAYOUT_COMPACT)
+        path = str(tmp_path / "layouts.json")
+        ctx.autotune_save(path)
+    with pg.Context(0) as c2:
+        c2.set_layout(gs, 1, 0)
+        c2.autotune_load(path)
+        assert c2.get_layout(gs) == (1, pg.LAYOUT_COMPACT)
+
+
 def test_layout_occupancy():
     """pgamd_layout_occupancy: resident SOR workgroups per CU at every ghost size and layout, consistent with the
     layout's patches per wavefront; at ghost size 5 and one patch per wavefront, LDS (the colour-split cube) and
