@@ -280,14 +280,20 @@ def test_guess_start_without_a_guess_array_is_refused():
         assert np.array_equal(out.cpu().numpy(), zero)        # the candidates ran the zero start
 
 
-def test_nonfinite_density_flagged():
-    gs = 2
+@pytest.mark.parametrize("gs", [2, 5])
+def test_nonfinite_density_flagged(gs):
+    """A NaN density cell makes its patch run to Max_Iter with the non-finite flag; the other patches of the
+    batch converge as usual (ghost size 5 runs the compact layout, with the cell in an LDS-held Const*rho)."""
     rho, cpot, flu = _mixed_inputs(16, gs, seed=99)
     rho[5, 3, 3, 3] = np.nan
+    if gs == 5:
+        rho[9, 14, 2, 7] = np.inf
     pot_g, _, it, st = pg.solve_host(rho, cpot, flu, gs=gs, term_mode=pg.TERM_RELRES, init_mode=pg.INIT_ZERO,
                                      sor_min_iter=5, sor_max_iter=80, sor_omega=1.6, poi_coeff=POI, graacc=False)
-    assert st[5] & pg.STATUS_NONFINITE and it[5] == 80
-    assert np.all(st[np.arange(16) != 5] == pg.STATUS_CONVERGED)
+    bad = [5, 9] if gs == 5 else [5]
+    for b in bad:
+        assert st[b] & pg.STATUS_NONFINITE and it[b] == 80
+    assert np.all(st[~np.isin(np.arange(16), bad)] == pg.STATUS_CONVERGED)
 
 
 def test_repeat_is_bitwise_deterministic():
