@@ -276,8 +276,10 @@ struct SplitAddr {
 // the Dirichlet cell x = 17; the Dirichlet rows y = 0 and 17 keep their 16 interior cells (x = 2m+1 at A+m,
 // x = 2m+2 at B+m).  The z = 0 and 17 planes stay in the lanes' registers.  16 KB per patch instead of the
 // split layout's 25.9 KB: eight resident patches per CU instead of six.  The 4 KB after the cube hold the
-// lane's Const*rho of its top four planes ([cell][lane], conflict-free): with them out of the registers the
-// sweep fits 256 VGPRs without spilling, and eight 20 KB patches fill the 160 KB of LDS exactly.
+// lane's Const*rho of its top four planes ([cell][lane], conflict-free), and eight 20 KB patches fill the 160 KB
+// of LDS exactly.  With those 16 values out of the registers the kernel still uses all 256 VGPRs and spills
+// 140 bytes per lane (set-up values, and a few values reloaded once per iteration), but no longer reloads
+// spilled Const*rho values all through each half-sweep (9% faster, measured).
 // Banks: the 32 lanes of an LDS cycle (yb = 0..7, four values of m) read rows 20*yb + m + const, 28 of the 32
 // four-byte banks for a y-neighbour read; A and B sit on the 4 banks left free (23..26 mod 32 for y = 0, 1..4
 // for y = 17).  kPS = 256 is a multiple of 64 floats, so two planes pair into one ds_read2st64 from the
@@ -296,9 +298,10 @@ struct Compact {
   }
 };
 
-// Five lane registers address every operand: the lane's row base B = 20*yb + m (x-neighbours at B + 10*yy +
-// 2*xx, its own slots at B + 10*yy + 1, compile-time offsets from one register) and the y-neighbour bases of
-// the two rows, which move to the Dirichlet rows for the lanes at yb = 0 (below) and yb = 7 (above).
+// Six lane registers address every operand: the lane's row base B = 20*yb + m (x-neighbours at B + 10*yy +
+// 2*xx, its own slots at B + 10*yy + 1, compile-time offsets from one register), the y-neighbour bases of the
+// two rows, which move to the Dirichlet rows for the lanes at yb = 0 (below) and yb = 7 (above), and the base
+// of the lane's Const*rho slots.
 template <int GS>
 struct CompactAddr {
   static constexpr int kLook = 4;
