@@ -101,6 +101,14 @@ __device__ __forceinline__ void load_weights(const u16* __restrict__ W, bf16x8 (
   }
 }
 
+// Step c of this lane's row of the forward layout (row j = lane%16 of [16][480]).
+__device__ __forceinline__ bf16x8 load_weight_step(const u16* __restrict__ W, int c) {
+  const int j = threadIdx.x & 15, kq = 8 * ((threadIdx.x & 63) >> 4);
+  Pack8 p;
+  p.u = *reinterpret_cast<const uint4*>(W + j * KF + c * 32 + kq);
+  return p.v;
+}
+
 // B operand of the forward layout (row j = lane%16 of [16][480]) for the 15 steps.
 __device__ __forceinline__ void load_weights_fwd(const u16* __restrict__ W, bf16x8 (&wreg)[KZ]) {
   const int j = threadIdx.x & 15, kq = 8 * ((threadIdx.x & 63) >> 4);
@@ -590,11 +598,16 @@ static_assert(kCoarseMax * 4 <= kActVec * 16 && kInteriorMax * 4 <= kActVec * 16
               "staging must fit a buffer");
 static_assert(kFusedBlocksPerCU * sizeof(FusedLds) <= 160 * 1024, "workgroups per CU");
 
+// The biases of layer l for this lane's output channels 4*(lane/16)..+3.
+__device__ __forceinline__ void load_bias(const float* __restrict__ P, int l, float4& b) {
+  b = *reinterpret_cast<const float4*>(P + (size_t)l * kCnnLayerParams + 16 * K + 4 * ((threadIdx.x & 63) >> 4));
+}
+
 // Layer l's weights (A operand) and the biases of this lane's output channels 4*(lane/16)..+3.
 __device__ __forceinline__ void load_layer(const u16* __restrict__ Wb, const float* __restrict__ P, int l,
                                            bf16x8 (&w)[KZ], float4& b) {
   load_weights_fwd(Wb + (size_t)l * 16 * KF, w);
-  b = *reinterpret_cast<const float4*>(P + (size_t)l * kCnnLayerParams + 16 * K + 4 * ((threadIdx.x & 63) >> 4));
+  load_bias(P, l, b);
 }
 
 // This lane's A-operand offsets (bf16 elements, relative to its row voxel in an input plane) for the five
@@ -671,9 +684,13 @@ __device__ __forceinline__ void conv_store(const u16* in, u16* out, const floatx
 // z - 1 (dz = 2); output plane o accumulates its steps in the order of conv_fwd_kernel (dz = 0, 1, 2; s = 0..4
 // within each) without the steps of the zero planes z = -1 and 8.  The operands of plane z + 1 are requested
 // before the MFMAs of plane z, so their LDS latency hides behind those.
+// Wn (not null): the next layer's weights are requested into w as the current ones die: the dz = 0 steps before
+// the last input plane (which feeds no plane z + 1), the others after its MFMAs.  The next layer then does not
+// start with the whole round trip, and no extra registers are needed: 4% of the kernel, measured.  (Leaving
+// the reloads out of a timing copy saves 11%: the rest is the bandwidth of 15 KB of weights per wave and layer.)
 template <int GS, bool CUBE, int MODE>
-__device__ __forceinline__ void conv_layer(const u16* in, u16* out, const bf16x8 (&w)[KZ], const float4& b,
-                                           const int (&off)[5], const GuessOut& go) {
+__device__ __forceinline__ void conv_layer(const u16* in, u16* out, bf16x8 (&w)[KZ], const float4& b,
+                                           const int (&off)[5], const GuessOut& go, const u16* __restrict__ Wn) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15;
   const u16* col = in + ((wave + 4 * (i >> 3)) * S + (i & 7)) * 8;   // padded row = output row + dy
   const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
@@ -691,11 +708,19 @@ __device__ __forceinline__ void conv_layer(const u16* in, u16* out, const bf16x8
       for (int s = 0; s < 5; s++) nx[s].u = *reinterpret_cast<const uint4*>(col + (z + 1) * kPlaneV + off[s]);
     }
     floatx4 a2 = zero;            // output plane z + 1
+    if (z + 1 == So && Wn) {
+#pragma unroll
+      for (int c = 0; c < 5; c++) w[c] = load_weight_step(Wn, c);
+    }
 #pragma unroll
     for (int s = 0; s < 5; s++) {
       if (z + 1 < So) a2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[s], a[s].v, a2, 0, 0, 0);
       a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[5 + s], a[s].v, a1, 0, 0, 0);
       if (z >= 1) a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[10 + s], a[s].v, a0, 0, 0, 0);
+    }
+    if (z + 1 == So && Wn) {
+#pragma unroll
+      for (int c = 5; c < KZ; c++) w[c] = load_weight_step(Wn, c);
     }
     if (z >= 1) conv_store<GS, CUBE, MODE>(in, out, a0, b, z - 1, go);
     a0 = a1;
@@ -826,23 +851,26 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   pair_offsets(off);
   __syncthreads();
   GuessOut go{s, reinterpret_cast<float*>(L.X)};   // X is free during the output layer (layer 5 read it)
-  conv_layer<GS, CUBE, 0>(X0, A, w, b, off, go);   // layer 1: packed input -> A
+  // each layer requests the next one's weights into w as it finishes; the bias follows it (first used after
+  // the next layer's first plane of MFMAs)
+  auto wnext = [&](int l) { return Wb + (size_t)l * 16 * KF; };
+  conv_layer<GS, CUBE, 0>(X0, A, w, b, off, go, wnext(1));   // layer 1: packed input -> A
+  load_bias(P, 1, b);
   __syncthreads();   // every wave is done reading the packed input
   u16* Bb = X0;
-  load_layer(Wb, P, 1, w, b);
-  conv_layer<GS, CUBE, 1>(A, Bb, w, b, off, go);   // layers 2-5: A -> B -> A -> B -> A
+  conv_layer<GS, CUBE, 1>(A, Bb, w, b, off, go, wnext(2));   // layers 2-5: A -> B -> A -> B -> A
+  load_bias(P, 2, b);
   __syncthreads();
-  load_layer(Wb, P, 2, w, b);
-  conv_layer<GS, CUBE, 1>(Bb, A, w, b, off, go);
+  conv_layer<GS, CUBE, 1>(Bb, A, w, b, off, go, wnext(3));
+  load_bias(P, 3, b);
   __syncthreads();
-  load_layer(Wb, P, 3, w, b);
-  conv_layer<GS, CUBE, 1>(A, Bb, w, b, off, go);
+  conv_layer<GS, CUBE, 1>(A, Bb, w, b, off, go, wnext(4));
+  load_bias(P, 4, b);
   __syncthreads();
-  load_layer(Wb, P, 4, w, b);
-  conv_layer<GS, CUBE, 1>(Bb, A, w, b, off, go);
+  conv_layer<GS, CUBE, 1>(Bb, A, w, b, off, go, wnext(5));
+  load_bias(P, 5, b);
   __syncthreads();
-  load_layer(Wb, P, 5, w, b);
-  conv_layer<GS, CUBE, 2>(A, nullptr, w, b, off, go);   // layer 6: s * D into X
+  conv_layer<GS, CUBE, 2>(A, nullptr, w, b, off, go, nullptr);   // layer 6: s * D into X
   __syncthreads();
   // the guess, interp + s * D, by interior columns (whole rows per wave)
   if (colv) {
