@@ -41,16 +41,17 @@ hipError_t cnn_conv_bwd_weight(const uint16_t* X, const uint16_t* dH, float* wpa
 hipError_t cnn_reduce_parts(const float* part, int parts, int len, float* dst, hipStream_t s);
 hipError_t cnn_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
                     float c1, float c2, hipStream_t s);
-// Wb: [6][16][kCnnKF] forward layout; WT: [6][16][432] per-tap transposed (input-gradient pass)
-hipError_t cnn_weight_copies(const float* P, uint16_t* Wb, uint16_t* WT, hipStream_t s);
+// Wb: [6][16][kCnnKF] forward layout; WT: [6][16][432] per-tap transposed (input-gradient pass); WL: the forward
+// layout lane-major, [6][15 steps][64 lanes][8], what lane l reads at step c (the fused kernel's A operand)
+hipError_t cnn_weight_copies(const float* P, uint16_t* Wb, uint16_t* WT, uint16_t* WL, hipStream_t s);
 hipError_t cnn_prep(const float* rho, const float* cpot, int n, int gs, float cst, int int_scheme, uint16_t* x,
                     float* scale, hipStream_t s);
 // Interpolation, packing, the six layers and the guess in one launch (activations in LDS).  cube = 0: guess of
 // the interior [n][R^3]; cube = 1: the whole solve cube [n][N^3], the Dirichlet layer being the interpolation.
-hipError_t cnn_fused_guess(const float* rho, const float* cpot, const uint16_t* Wb, const float* P, int n, int gs,
+hipError_t cnn_fused_guess(const float* rho, const float* cpot, const uint16_t* WL, const float* P, int n, int gs,
                            float cst, int int_scheme, float* guess, int cube, hipStream_t s);
 struct PatchDesc;
-hipError_t cnn_fused_guess_mixed(const float* rho, const float* cpot, const PatchDesc* desc, const uint16_t* Wb,
+hipError_t cnn_fused_guess_mixed(const float* rho, const float* cpot, const PatchDesc* desc, const uint16_t* WL,
                                  const float* P, int n, float cst, int int_scheme, float* guess, hipStream_t s);
 hipError_t cnn_guess(const float* cpot, const float* D, const float* scale, int n, int gs, int int_scheme,
                      float* guess, hipStream_t s);

@@ -101,11 +101,10 @@ __device__ __forceinline__ void load_weights(const u16* __restrict__ W, bf16x8 (
   }
 }
 
-// Step c of this lane's row of the forward layout (row j = lane%16 of [16][480]).
-__device__ __forceinline__ bf16x8 load_weight_step(const u16* __restrict__ W, int c) {
-  const int j = threadIdx.x & 15, kq = 8 * ((threadIdx.x & 63) >> 4);
+// Step c of this lane's A operand from the lane-major copy WL of one layer ([15][64][8]).
+__device__ __forceinline__ bf16x8 load_weight_step(const u16* __restrict__ WL, int c) {
   Pack8 p;
-  p.u = *reinterpret_cast<const uint4*>(W + j * KF + c * 32 + kq);
+  p.u = *reinterpret_cast<const uint4*>(WL + ((size_t)c * 64 + (threadIdx.x & 63)) * 8);
   return p.v;
 }
 
@@ -414,16 +413,20 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, 
 }
 
 // BF16 copies of the six weight matrices: the forward layout Wb[l][co][c*32 + kk] (step c = dz*5 + s,
-// position 2s + kk/16, channel kk%16; zero at the pad position 9), and per-tap transposed
-// WT[l][ci][tap*16+co] = W[l][co][tap*16+ci] for the input-gradient pass.
-__global__ void weight_copies_kernel(const float* __restrict__ P, u16* __restrict__ Wb, u16* __restrict__ WT) {
+// position 2s + kk/16, channel kk%16; zero at the pad position 9), the same values lane-major
+// WL[l][c][lane][8] (lane = co + 16*(kk/8) reads kk%8 = 0..7 of its row: one contiguous 1 KB load per step),
+// and per-tap transposed WT[l][ci][tap*16+co] = W[l][co][tap*16+ci] for the input-gradient pass.
+__global__ void weight_copies_kernel(const float* __restrict__ P, u16* __restrict__ Wb, u16* __restrict__ WT,
+                                     u16* __restrict__ WL) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kCnnLayers * 16 * KF) {
     const int l = i / (16 * KF), r = i % (16 * KF);
     const int co = r / KF, k = r % KF, c = k / 32, kk = k % 32;
     const int dz = c / 5, pos = 2 * (c % 5) + (kk >> 4);
     const float* W = P + (size_t)l * kCnnLayerParams;
-    Wb[i] = pos < 9 ? f2bf(W[co * K + (dz * 9 + pos) * CH + (kk & 15)]) : (u16)0;
+    const u16 v = pos < 9 ? f2bf(W[co * K + (dz * 9 + pos) * CH + (kk & 15)]) : (u16)0;
+    Wb[i] = v;
+    WL[(((size_t)l * KZ + c) * 64 + co + 16 * (kk >> 3)) * 8 + (kk & 7)] = v;
   }
   if (i < kCnnLayers * 16 * K) {
     const int l = i / (16 * K), r = i % (16 * K);
@@ -603,10 +606,12 @@ __device__ __forceinline__ void load_bias(const float* __restrict__ P, int l, fl
   b = *reinterpret_cast<const float4*>(P + (size_t)l * kCnnLayerParams + 16 * K + 4 * ((threadIdx.x & 63) >> 4));
 }
 
-// Layer l's weights (A operand) and the biases of this lane's output channels 4*(lane/16)..+3.
-__device__ __forceinline__ void load_layer(const u16* __restrict__ Wb, const float* __restrict__ P, int l,
+// Layer l's weights (A operand, from the lane-major copy WL) and the biases of this lane's output channels
+// 4*(lane/16)..+3.
+__device__ __forceinline__ void load_layer(const u16* __restrict__ WL, const float* __restrict__ P, int l,
                                            bf16x8 (&w)[KZ], float4& b) {
-  load_weights_fwd(Wb + (size_t)l * 16 * KF, w);
+#pragma unroll
+  for (int c = 0; c < KZ; c++) w[c] = load_weight_step(WL + (size_t)l * 16 * KF, c);
   load_bias(P, l, b);
 }
 
@@ -762,7 +767,7 @@ __device__ __forceinline__ float interp_cube_lds(const float* cs, int int_scheme
 }
 
 template <int GS, bool CUBE, bool KEEP>
-__device__ void fused_patch(const float* __restrict__ rho, const float* __restrict__ cpot, const u16* __restrict__ Wb,
+__device__ void fused_patch(const float* __restrict__ rho, const float* __restrict__ cpot, const u16* __restrict__ WL,
                             const float* __restrict__ P, float cst, int int_scheme, float* __restrict__ guess,
                             FusedLds& L) {
   constexpr Geo g = geo_of(GS);
@@ -844,7 +849,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   }
   bf16x8 w[KZ];
   float4 b;
-  load_layer(Wb, P, 0, w, b);
+  load_layer(WL, P, 0, w, b);
   __syncthreads();   // the packed input is complete; the cube is dead
   zero_ring(L.C);
   int off[5];
@@ -853,7 +858,7 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
   GuessOut go{s, reinterpret_cast<float*>(L.X)};   // X is free during the output layer (layer 5 read it)
   // each layer requests the next one's weights into w as it finishes; the bias follows it (first used after
   // the next layer's first plane of MFMAs)
-  auto wnext = [&](int l) { return Wb + (size_t)l * 16 * KF; };
+  auto wnext = [&](int l) { return WL + (size_t)l * 16 * KF; };
   conv_layer<GS, CUBE, 0>(X0, A, w, b, off, go, wnext(1));   // layer 1: packed input -> A
   load_bias(P, 1, b);
   __syncthreads();   // every wave is done reading the packed input
@@ -885,14 +890,14 @@ __device__ void fused_patch(const float* __restrict__ rho, const float* __restri
 }
 
 template <bool CUBE, bool KEEP>
-__device__ __forceinline__ void fused_dispatch(int gs, const float* r, const float* c, const u16* Wb, const float* P,
+__device__ __forceinline__ void fused_dispatch(int gs, const float* r, const float* c, const u16* WL, const float* P,
                                                float cst, int int_scheme, float* out, FusedLds& L) {
   switch (gs) {
-    case 1: fused_patch<1, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 2: fused_patch<2, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 3: fused_patch<3, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    case 4: fused_patch<4, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
-    default: fused_patch<5, CUBE, KEEP>(r, c, Wb, P, cst, int_scheme, out, L); break;
+    case 1: fused_patch<1, CUBE, KEEP>(r, c, WL, P, cst, int_scheme, out, L); break;
+    case 2: fused_patch<2, CUBE, KEEP>(r, c, WL, P, cst, int_scheme, out, L); break;
+    case 3: fused_patch<3, CUBE, KEEP>(r, c, WL, P, cst, int_scheme, out, L); break;
+    case 4: fused_patch<4, CUBE, KEEP>(r, c, WL, P, cst, int_scheme, out, L); break;
+    default: fused_patch<5, CUBE, KEEP>(r, c, WL, P, cst, int_scheme, out, L); break;
   }
 }
 
@@ -900,7 +905,7 @@ __device__ __forceinline__ void fused_dispatch(int gs, const float* r, const flo
 template <bool CUBE>
 __global__ void __launch_bounds__(kFT, kFusedBlocksPerCU) fused_guess_kernel(const float* __restrict__ rho,
                                                              const float* __restrict__ cpot,
-                                                             const u16* __restrict__ Wb, const float* __restrict__ P,
+                                                             const u16* __restrict__ WL, const float* __restrict__ P,
                                                              int npatch, int gs, float cst, int int_scheme,
                                                              float* __restrict__ guess) {
   __shared__ FusedLds L;
@@ -910,7 +915,7 @@ __global__ void __launch_bounds__(kFT, kFusedBlocksPerCU) fused_guess_kernel(con
   const float* r = rho + (size_t)p * g.R * g.R * g.R;
   const float* c = cpot + (size_t)p * g.C * g.C * g.C;
   float* out = guess + (size_t)p * (CUBE ? g.N * g.N * g.N : g.R * g.R * g.R);
-  fused_dispatch<CUBE, true>(gs, r, c, Wb, P, cst, int_scheme, out, L);
+  fused_dispatch<CUBE, true>(gs, r, c, WL, P, cst, int_scheme, out, L);
 }
 
 // Mixed ghost sizes: patch p reads its ghost size and packed offsets from the plan's descriptors; the
@@ -918,14 +923,14 @@ __global__ void __launch_bounds__(kFT, kFusedBlocksPerCU) fused_guess_kernel(con
 __global__ void __launch_bounds__(kFT, kFusedBlocksPerCU) fused_guess_mixed_kernel(const float* __restrict__ rho,
                                                                    const float* __restrict__ cpot,
                                                                    const PatchDesc* __restrict__ desc,
-                                                                   const u16* __restrict__ Wb,
+                                                                   const u16* __restrict__ WL,
                                                                    const float* __restrict__ P, int npatch, float cst,
                                                                    int int_scheme, float* __restrict__ guess) {
   __shared__ FusedLds L;
   const int p = blockIdx.x;
   if (p >= npatch) return;
   const PatchDesc d = desc[p];
-  fused_dispatch<false, false>(d.gs, rho + d.rho_off, cpot + d.cpot_off, Wb, P, cst, int_scheme, guess + d.rho_off, L);
+  fused_dispatch<false, false>(d.gs, rho + d.rho_off, cpot + d.cpot_off, WL, P, cst, int_scheme, guess + d.rho_off, L);
 }
 
 }  // namespace cnn
@@ -989,9 +994,9 @@ hipError_t cnn_adam(float* p, const float* g, float* m, float* v, int n, float l
   return hipGetLastError();
 }
 
-hipError_t cnn_weight_copies(const float* P, uint16_t* Wb, uint16_t* WT, hipStream_t s) {
-  const int n = kCnnLayers * 16 * KF;   // covers both copies (KF > K)
-  hipLaunchKernelGGL(weight_copies_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, Wb, WT);
+hipError_t cnn_weight_copies(const float* P, uint16_t* Wb, uint16_t* WT, uint16_t* WL, hipStream_t s) {
+  const int n = kCnnLayers * 16 * KF;   // covers every copy (KF > K)
+  hipLaunchKernelGGL(weight_copies_kernel, dim3((n + 255) / 256), dim3(256), 0, s, P, Wb, WT, WL);
   return hipGetLastError();
 }
 
@@ -1003,20 +1008,20 @@ hipError_t cnn_prep(const float* rho, const float* cpot, int n, int gs, float cs
   return hipGetLastError();
 }
 
-hipError_t cnn_fused_guess(const float* rho, const float* cpot, const uint16_t* Wb, const float* P, int n, int gs,
+hipError_t cnn_fused_guess(const float* rho, const float* cpot, const uint16_t* WL, const float* P, int n, int gs,
                            float cst, int int_scheme, float* guess, int cube, hipStream_t s) {
   if (cube)
-    hipLaunchKernelGGL(fused_guess_kernel<true>, dim3(n), dim3(kFT), 0, s, rho, cpot, Wb, P, n, gs, cst, int_scheme,
+    hipLaunchKernelGGL(fused_guess_kernel<true>, dim3(n), dim3(kFT), 0, s, rho, cpot, WL, P, n, gs, cst, int_scheme,
                        guess);
   else
-    hipLaunchKernelGGL(fused_guess_kernel<false>, dim3(n), dim3(kFT), 0, s, rho, cpot, Wb, P, n, gs, cst, int_scheme,
+    hipLaunchKernelGGL(fused_guess_kernel<false>, dim3(n), dim3(kFT), 0, s, rho, cpot, WL, P, n, gs, cst, int_scheme,
                        guess);
   return hipGetLastError();
 }
 
-hipError_t cnn_fused_guess_mixed(const float* rho, const float* cpot, const PatchDesc* desc, const uint16_t* Wb,
+hipError_t cnn_fused_guess_mixed(const float* rho, const float* cpot, const PatchDesc* desc, const uint16_t* WL,
                                  const float* P, int n, float cst, int int_scheme, float* guess, hipStream_t s) {
-  hipLaunchKernelGGL(fused_guess_mixed_kernel, dim3(n), dim3(kFT), 0, s, rho, cpot, desc, Wb, P, n, cst,
+  hipLaunchKernelGGL(fused_guess_mixed_kernel, dim3(n), dim3(kFT), 0, s, rho, cpot, desc, WL, P, n, cst,
                      int_scheme, guess);
   return hipGetLastError();
 }

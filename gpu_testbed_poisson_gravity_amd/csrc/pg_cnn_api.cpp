@@ -42,6 +42,7 @@ struct pgamd_cnn {
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
   uint16_t* wb = nullptr;   // BF16 W1..W6 in the forward layout [16][480]
   uint16_t* wt = nullptr;   // BF16 W1..W6 transposed per tap (input-gradient pass)
+  uint16_t* wl = nullptr;   // BF16 W1..W6 forward layout, lane-major (the fused warm-start kernel)
   // workspace
   int cap = 0;              // samples
   bool train_ws = false;
@@ -116,7 +117,7 @@ int ensure_ws(pgamd_cnn* n, int count, bool train) {
   } while (0)
 
 int refresh_copies(pgamd_cnn* n, hipStream_t s) {
-  CK(cnn_weight_copies(n->p, n->wb, n->wt, s));
+  CK(cnn_weight_copies(n->p, n->wb, n->wt, n->wl, s));
   return PGAMD_OK;
 }
 
@@ -197,7 +198,7 @@ int pgamd_cnn_create(pgamd_context* ctx, uint64_t seed, pgamd_cnn** out) {
   if (hipSetDevice(n->device) != hipSuccess || hipMalloc(&n->p, kCnnParams * 4) != hipSuccess ||
       hipMalloc(&n->g, kCnnParams * 4) != hipSuccess || hipMalloc(&n->m, kCnnParams * 4) != hipSuccess ||
       hipMalloc(&n->v, kCnnParams * 4) != hipSuccess || hipMalloc(&n->wb, kCnnLayers * kWF * 2) != hipSuccess ||
-      hipMalloc(&n->wt, kCnnLayers * kW * 2) != hipSuccess) {
+      hipMalloc(&n->wt, kCnnLayers * kW * 2) != hipSuccess || hipMalloc(&n->wl, kCnnLayers * kWF * 2) != hipSuccess) {
     pgamd_cnn_destroy(n);
     return set_error(PGAMD_ERR_NOMEM, "CNN parameter allocation failed");
   }
@@ -235,6 +236,7 @@ int pgamd_cnn_destroy(pgamd_cnn* n) {
   hipFree(n->v);
   hipFree(n->wb);
   hipFree(n->wt);
+  hipFree(n->wl);
   for (hipEvent_t e : n->ev)
     if (e) hipEventDestroy(e);
   delete n;
@@ -328,7 +330,7 @@ int pgamd_cnn_warm_start(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho, c
   hipSetDevice(n->device);
   const float cst = a->poi_coeff * a->dh * a->dh;
   // one launch: interpolation, packing, the six layers in LDS and the guess
-  CK(cnn_fused_guess(d_rho, d_pot_in, n->wb, n->p, npatch, gs, cst, a->int_scheme, d_guess, 0, (hipStream_t)stream));
+  CK(cnn_fused_guess(d_rho, d_pot_in, n->wl, n->p, npatch, gs, cst, a->int_scheme, d_guess, 0, (hipStream_t)stream));
   return PGAMD_OK;
 }
 
@@ -341,7 +343,7 @@ int pgamd_cnn_warm_start_cube(pgamd_context* ctx, pgamd_cnn* n, const float* d_r
   const int gs = ctx_options(ctx).pot_ghost_size;
   hipSetDevice(n->device);
   const float cst = a->poi_coeff * a->dh * a->dh;
-  CK(cnn_fused_guess(d_rho, d_pot_in, n->wb, n->p, npatch, gs, cst, a->int_scheme, d_guess, 1, (hipStream_t)stream));
+  CK(cnn_fused_guess(d_rho, d_pot_in, n->wl, n->p, npatch, gs, cst, a->int_scheme, d_guess, 1, (hipStream_t)stream));
   return PGAMD_OK;
 }
 
@@ -356,7 +358,7 @@ int pgamd_cnn_warm_start_mixed(pgamd_context* ctx, pgamd_cnn* n, pgamd_plan* pla
   if (np == 0) return PGAMD_OK;
   hipSetDevice(n->device);
   const float cst = a->poi_coeff * a->dh * a->dh;
-  CK(cnn_fused_guess_mixed(d_rho, d_pot_in, plan_device_desc(plan), n->wb, n->p, np, cst, a->int_scheme, d_guess,
+  CK(cnn_fused_guess_mixed(d_rho, d_pot_in, plan_device_desc(plan), n->wl, n->p, np, cst, a->int_scheme, d_guess,
                            (hipStream_t)stream));
   return PGAMD_OK;
 }
@@ -394,7 +396,7 @@ int pgamd_cnn_solve_device(pgamd_context* ctx, pgamd_cnn* n, const float* d_rho,
   const int chunk = (npatch + nchunk - 1) / nchunk;
   for (int i = 0, off = 0; off < npatch; i++, off += chunk) {
     const int m = std::min(chunk, npatch - off);
-    CK(cnn_fused_guess(d_rho + off * R3, d_pot_in + off * C3, n->wb, n->p, m, o.pot_ghost_size, cst, a->int_scheme,
+    CK(cnn_fused_guess(d_rho + off * R3, d_pot_in + off * C3, n->wl, n->p, m, o.pot_ghost_size, cst, a->int_scheme,
                        d_guess + off * Q3, cube, s1));
     CK(hipEventRecord(n->ev[i], s1));
     CK(hipStreamWaitEvent(s0, n->ev[i], 0));
