@@ -1,7 +1,7 @@
 """Train the warm-start CNN on patches solved by this library's own SOR solver, then evaluate it.
 
-    python -m gpu_testbed_poisson_gravity_amd.train_warm_start --patches 200000 --steps 20000 --batch 256 \
-        --lr 1e-3 --res-weight 1.0 --out gpu_testbed_poisson_gravity_amd/weights/warm_start_gs5.npy
+    python -m gpu_testbed_poisson_gravity_amd.train_warm_start --patches 400000 --steps 120000 --batch 256 \
+        --lr 1e-3 --decay-frac 0.5 --res-weight 1.0 --out gpu_testbed_poisson_gravity_amd/weights/warm_start_gs5.npy
 
 Data: ghost size 5 (18^3 solve cube), density fields half sums of 1..8 Plummer spheres and half lognormal
 (sigma 1.5, correlation length 3 cells), generated on the host cores before the GPU is touched.  Targets:
@@ -143,6 +143,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--decay-frac", type=float, default=0.25,
+                    help="fraction of the steps, at the end, over which the rate decays linearly to lr/10")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--eval-patches", type=int, default=16384)
     ap.add_argument("--time-limit", type=float, default=1e9, help="stop training after this many seconds")
@@ -152,6 +154,8 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="start from these weights (.npy)")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights", "warm_start_gs5.npy"))
     a = ap.parse_args(argv)
+    if not 0.0 < a.decay_frac <= 1.0:
+        ap.error("--decay-frac must lie in (0, 1]")
     t_start = time.time()
 
     # host-side generation first, before anything initialises the GPU
@@ -196,9 +200,10 @@ def main(argv=None):
         idx = torch.randint(0, n, (a.batch,), device="cuda", generator=gen)
         xb, tb = x0.index_select(0, idx), target.index_select(0, idx)
         want = step % 100 == 0 or step == a.steps - 1
-        # constant rate, then a linear decay to a tenth over the last quarter of the steps
+        # constant rate, then a linear decay to a tenth over the last decay_frac of the steps
         frac = step / max(a.steps - 1, 1)
-        lr = a.lr if frac < 0.75 else a.lr * (1.0 - 0.9 * (frac - 0.75) / 0.25)
+        d0 = 1.0 - a.decay_frac
+        lr = a.lr if frac < d0 else a.lr * (1.0 - 0.9 * (frac - d0) / a.decay_frac)
         loss = net.train_step(xb, tb, a.batch, lr, want_loss=want)
         done = step + 1
         if want:
@@ -228,7 +233,8 @@ def main(argv=None):
               f"{k['ms']:.2f} ms", flush=True)
     record = {"patches": n, "lognormal_frac": a.lognormal_frac, "steps": done, "batch": a.batch, "lr": a.lr,
               "res_weight": a.res_weight,
-              "lr_schedule": "constant, then linear decay to lr/10 over the last quarter", "target_rms": tscale,
+              "lr_schedule": f"constant, then linear decay to lr/10 over the last {a.decay_frac:g} of the steps",
+              "init": a.init, "target_rms": tscale,
               "seed": a.seed, "train_seconds": train_s, "ms_per_step": 1e3 * train_s / max(done, 1),
               "loss_curve": curve, "target_solver": {"rel_tol": 1e-6, "mean_iters": float(it_t.mean())},
               "eval_patches": a.eval_patches, "eval": ev, "device": ctx.device_info()}
