@@ -1,7 +1,12 @@
 """Train the warm-start CNN on patches solved by this library's own SOR solver, then evaluate it.
 
     python -m gpu_testbed_poisson_gravity_amd.train_warm_start --patches 400000 --steps 120000 --batch 256 \
-        --lr 1e-3 --decay-frac 0.5 --res-weight 1.0 --out gpu_testbed_poisson_gravity_amd/weights/warm_start_gs5.npy
+        --lr 1e-3 --decay-frac 0.5 --res-weight 1.0 --out stage1.npy
+    python -m gpu_testbed_poisson_gravity_amd.train_warm_start --patches 400000 --steps 120000 --batch 256 \
+        --lr 3e-4 --decay-frac 0.5 --res-weight 1.0 --seed 2 --init stage1.npy \
+        --out gpu_testbed_poisson_gravity_amd/weights/warm_start_gs5.npy
+
+(the shipped weights: a first run, then a second from its weights on fresh patches at a lower rate)
 
 Data: ghost size 5 (18^3 solve cube), density fields half sums of 1..8 Plummer spheres and half lognormal
 (sigma 1.5, correlation length 3 cells), generated on the host cores before the GPU is touched.  Targets:
