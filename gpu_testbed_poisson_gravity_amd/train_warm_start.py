@@ -141,6 +141,14 @@ def evaluate(pg, ctx, net, rho, cpot, tol=1e-5, fallback_tol=1e-2, repeats=3):
     return res
 
 
+def learning_rate(step: int, steps: int, lr: float, decay_frac: float) -> float:
+    """Constant rate lr, then a linear decay to lr/10 over the last decay_frac of the steps (for steps > 1
+    the last step runs at lr/10)."""
+    frac = step / max(steps - 1, 1)
+    d0 = 1.0 - decay_frac
+    return lr if frac < d0 else lr * (1.0 - 0.9 * (frac - d0) / decay_frac)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--patches", type=int, default=200000)
@@ -205,10 +213,7 @@ def main(argv=None):
         idx = torch.randint(0, n, (a.batch,), device="cuda", generator=gen)
         xb, tb = x0.index_select(0, idx), target.index_select(0, idx)
         want = step % 100 == 0 or step == a.steps - 1
-        # constant rate, then a linear decay to a tenth over the last decay_frac of the steps
-        frac = step / max(a.steps - 1, 1)
-        d0 = 1.0 - a.decay_frac
-        lr = a.lr if frac < d0 else a.lr * (1.0 - 0.9 * (frac - d0) / a.decay_frac)
+        lr = learning_rate(step, a.steps, a.lr, a.decay_frac)
         loss = net.train_step(xb, tb, a.batch, lr, want_loss=want)
         done = step + 1
         if want:
